@@ -1,0 +1,149 @@
+/*
+ * swd_abi.h - C ABI of libswd, the MI355X (gfx950) engine for the hot path of
+ * Skeletonized Wave-Equation Dispersion Spectrum Inversion (zhangchang271/SWD).
+ *
+ * Plain C types only (pointers and sizes, no torch/HIP types).  Unless stated
+ * otherwise, array arguments are host memory in MATLAB layout: column-major,
+ * first index fastest (depth z for model grids, time t for gathers).
+ * Every entry point returns SWD_OK (0) or a negative SWD_ERR_* code; the
+ * message of the last failure on the calling thread is swd_last_error().
+ * Entry points are thread-safe; each call uses its own HIP stream.
+ */
+#ifndef SWD_ABI_H
+#define SWD_ABI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWD_OK 0
+#define SWD_ERR_ARG (-2)      /* bad descriptor, or input the reference MEX would turn into UB */
+#define SWD_ERR_CFL (-3)      /* time step violates the staggered-grid stability bound */
+#define SWD_ERR_NOMEM (-4)    /* device / host allocation failed, or admission refused */
+#define SWD_ERR_HIP (-5)      /* HIP runtime error */
+#define SWD_ERR_PICK (-6)     /* picking produced fewer than two kept columns (interp1 would error) */
+#define SWD_ERR_NODEV (-7)    /* no usable gfx950 device */
+#define SWD_ERR_BUSY (-8)     /* handle in use / cancelled */
+
+/* Library identity and diagnostics. */
+const char *swd_version(void);
+const char *swd_last_error(void);
+int swd_device_info(int *n_devices, char *arch, int arch_len);
+
+/* ------------------------------------------------------------------------
+ * FD kernels with the MEX argument lists.
+ * ------------------------------------------------------------------------ */
+
+/*
+ * Forward modelling.  Replaces the MEX eigen_staggerfd_single
+ * (core/eigen_staggerfd_single.cpp:68, MexFunction::operator()) and, with
+ * double arrays, the binary-only eigen_staggerfd_double, as called by
+ * core/staggerfd_eigen.m:139 / core/staggerfd_eigend.m:139:
+ *   [seismo_u, seismo_w, S] = eigen_staggerfd_*(iv, temp, ca, cl, cm, cm1, b, b1, s)
+ * iv[16] = [nt nzbc nxbc dtx ng sz sx gz gx dg source_type fd_order in_wf nz nx format]
+ *          (sz, sx, gz, gx 1-based on the padded grid, cpp:98-118).
+ * temp..b1: nzbc x nxbc; s: nt.  cm is accepted and unused, as in the MEX.
+ * Outputs (caller-allocated): seismo_u, seismo_w nt x ng; fux, fuz, bwx, bwz
+ * nz x nx x (nt/in_wf) strain snapshots (the struct S of cpp:375-379; the
+ * format-2 shape nz x (nx*nt/in_wf) is the same memory).  seismo_u and the
+ * four snapshot pointers may be NULL to skip them (forward-only use).
+ */
+int swd_staggerfd_f32(const float *iv, const float *temp, const float *ca, const float *cl, const float *cm,
+                      const float *cm1, const float *b, const float *b1, const float *s, float *seismo_u,
+                      float *seismo_w, float *fux, float *fuz, float *bwx, float *bwz);
+int swd_staggerfd_f64(const double *iv, const double *temp, const double *ca, const double *cl, const double *cm,
+                      const double *cm1, const double *b, const double *b1, const double *s, double *seismo_u,
+                      double *seismo_w, double *fux, double *fuz, double *bwx, double *bwz);
+
+/*
+ * Adjoint propagation + imaging.  Replaces the MEX eigen_e2drtm_single
+ * (core/eigen_e2drtm_single.cpp:67) and the binary-only eigen_e2drtm_double,
+ * as called by core/e2drtm_eigen.m:122 / core/e2drtm_eigend.m:122:
+ *   [cl_img, cm_img, illum_div] = eigen_e2drtm_*(iv, temp, ca, cl, cm, cm1, b, b1, adj, fux, fuz, bwx, bwz)
+ * iv[14] = [nt nzbc nxbc dtx ng nbc gz gx dg fd_order in_wf nz nx dt] (cpp:69-84).
+ * adj: nt x ng adjoint source; snapshots nz x nx x (nt/in_wf).
+ * Outputs nz x nx.  Images accumulate in double on the device.
+ */
+int swd_e2drtm_f32(const float *iv, const float *temp, const float *ca, const float *cl, const float *cm,
+                   const float *cm1, const float *b, const float *b1, const float *adj, const float *fux,
+                   const float *fuz, const float *bwx, const float *bwz, float *cl_img, float *cm_img,
+                   float *illum_div);
+int swd_e2drtm_f64(const double *iv, const double *temp, const double *ca, const double *cl, const double *cm,
+                   const double *cm1, const double *b, const double *b1, const double *adj, const double *fux,
+                   const double *fuz, const double *bwx, const double *bwz, double *cl_img, double *cm_img,
+                   double *illum_div);
+
+/* Select the FD code path of the two entry points above:
+ * 0 = automatic (fused kernels when fd_order 22 and sources inside the
+ * update block, plain kernels otherwise), 1 = always plain.  Returns the
+ * previous value.  Used by tests and the unfused comparator. */
+int swd_set_fd_path(int mode);
+
+/*
+ * Model preparation of the wrappers: core/staggerfd_eigen.m:37-107 (with
+ * core/calparam.m:3-5, core/pad.m:3-9, core/damp_circle.m:5-67) for
+ * parameter_type 0 and a free surface (isfs 1; the wrapper leaves b, b1 and
+ * cm1 undefined otherwise).  vp, vs: nz x nx.  Outputs nzbc x nxbc with
+ * nzbc = nz + pad_top + nbc, nxbc = nx + 2 nbc, pad_top = (fd_order-20)/2+1.
+ * e2drtm_eigen.m:44-101 prepares the same arrays.
+ */
+int swd_prepare_model(const double *vp, const double *vs, int nz, int nx, int nbc, double dx, double dt,
+                      int fd_order, double *temp, double *ca, double *cl, double *cm, double *cm1, double *b,
+                      double *b1);
+
+/* Ricker wavelet, core/ricker.m:18-35 with nargin == 3 (w has nt samples). */
+int swd_ricker(double f, double dt, int nt, double *w);
+
+/* ------------------------------------------------------------------------
+ * Dispersion path (FP32 on the GPU: DFT-at-bins and slant stack on MFMA).
+ * ------------------------------------------------------------------------ */
+
+/*
+ * Linear Radon transform of one side of one shot gather.  Replaces
+ * core/RTr.m:2 (side 0) / core/RTl.m:2 (side 1) and their AD twins
+ * core/RTrAD.m:2 / core/RTlAD.m:2:
+ *   [ml, dataLen(, saveForBackward)] = RTx(seismo, is, df, dt, np, vmin, vmax, fmin, fmax, a, b, ds, offset, m, M)
+ * seismo: nt x ng (a = nt, b = ng).  Outputs: ml np x npair (column-max
+ * normalised |mm|, npair = nf-lf+1); optional mlr np x npair (|mm|, the AD
+ * field mlr/mll) and mm np x npair complex interleaved (re, im) = columns
+ * lf..nf of the AD field mm.  info[4] = {lf, nf, ccn, first trace (1-based)
+ * of window position 0}.  swd_radon takes double gathers (SWD.m),
+ * swd_radon_f32 single ones (SWD_single.m).
+ */
+int swd_radon(int side, const double *seismo, int nt, int ng, int is, double df, double dt, int np, double vmin,
+              double vmax, double fmin, double fmax, double ds, int offset, int m, int M, float *ml, int *dataLen,
+              float *mlr, float *mm, int *info);
+int swd_radon_f32(int side, const float *seismo, int nt, int ng, int is, double df, double dt, int np, double vmin,
+                  double vmax, double fmin, double fmax, double ds, int offset, int m, int M, float *ml,
+                  int *dataLen, float *mlr, float *mm, int *info);
+
+/*
+ * Dispersion pick and misfit.  Replaces core/LHDispPick.m:2
+ *   [res, cr_pre] = LHDispPick(ml, vmin, cr_0, method, ini)
+ * (method 1 = pickFDC1(ml, ini, 2, 4), core/pickFDC1.m:1; method 2 = column
+ * argmax).  ml: np x npair; cr_0, cr_pre: npair.  Returns SWD_ERR_PICK where
+ * MATLAB's interp1 would error (fewer than two kept columns).
+ */
+int swd_lhdisppick(const float *ml, int np, int npair, double vmin, const double *cr_0, int method, int ini,
+                   double *res, double *cr_pre);
+
+/*
+ * SoftMax adjoint source.  Replaces core/ADWDgrad_1.m:1
+ *   a_data_res = ADWDgrad_1(nt, ng, ns, npair, is, w, m, M, SoftArgNorm, gor, gol, space_M, sfbR, sfbL)
+ * The saveForBackward structs are passed as their fields: mm (np x npair
+ * complex interleaved), mlr/mll (np x npair), the window as its position-0
+ * trace pos (1-based: uxtposr(1), resp. uxtposl(end)) and length L, and the
+ * scalars lf, ccn; ll0 = 1i*2*pi*df*(1./(vmin:vmin+np-1))'*((m+(0:L-1))*ds)
+ * is rebuilt from np, vmin, df, ds, m.  space_M is 1..np in every column
+ * (SWD.m:133-136).  Branch guards as ADWDgrad_1.m:13/47.  a_data_res: nt x ng.
+ */
+int swd_adwdgrad1(int nt, int ng, int ns, int npair, int is, double w, int m, int M, double SoftArgNorm,
+                  const double *grad_outputr, const double *grad_outputl, const float *mmr, const float *mlr,
+                  int posr, int Lr, const float *mml, const float *mll, int posl, int Ll, int np, double vmin,
+                  double df, double ds, int lf, int ccn, float *a_data_res);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWD_ABI_H */

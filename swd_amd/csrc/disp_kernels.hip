@@ -1,0 +1,584 @@
+// Dispersion-spectrum kernels of the SWD hot path for gfx950 (MI355X):
+// trace DFT at the Radon bins, the per-frequency complex slant stack and its
+// adjoint (both on FP32 MFMA), spectrum normalisation, dispersion picking in
+// wavefront code, and the fused SoftMax (soft-argmax) backward.
+//
+// Reference: core/RTr.m:5-32 (RTl.m, RTrAD.m, RTlAD.m), core/pickFDC1.m:19-131,
+// core/LHDispPick.m:4-12, core/ADWDgrad_1.m:13-77.
+//
+// Column ("col") = one (shot, side) spectrum: col = side * nshots + shot,
+// side 0 = right window (RTr), 1 = left window (RTl, traces read flipped).
+// Frequency index k = 0..npair-1 is MATLAB column luoj = lf + k, i.e. DFT bin
+// c = lf - 1 + k and kernel frequency df * c.
+#include "swd_internal.h"
+#include "swd_disp.h"
+
+namespace swd {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4f mfma16x16x4(float a, float b, v4f c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Twiddle table W[t][k2] (nt_eff x 2 npair): cos and -sin of 2 pi c t / ccn,
+// c = lf - 1 + k, phase reduced exactly in integers (c t mod ccn).
+// ---------------------------------------------------------------------------
+__global__ void dft_table_kernel(float* W, int nt_eff, int npair, int c0, int ccn)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.y;
+    if (k >= npair || t >= nt_eff) return;
+    const long long r = ((long long)(c0 + k) * (long long)t) % ccn;
+    double s, c;
+    sincospi(2.0 * (double)r / (double)ccn, &s, &c);
+    W[(size_t)t * 2 * npair + k] = (float)c;
+    W[(size_t)t * 2 * npair + npair + k] = (float)(-s);
+}
+
+// ---------------------------------------------------------------------------
+// Strided batched FP32 GEMM on MFMA 16x16x4:
+//   C(b, m, n) = alpha * sum_k A(b, m, k) B(b, k, n)
+// 64x64 tile per 256-thread block, 4 wavefronts x (32x32), BK = 16.
+// ---------------------------------------------------------------------------
+constexpr int GBM = 64, GBN = 64, GBK = 16;
+
+__global__ __launch_bounds__(256) void gemm_f32_kernel(GemmArgs g)
+{
+    __shared__ float As[GBK][GBM + 4];
+    __shared__ float Bs[GBK][GBN + 4];
+    const int b = blockIdx.z;
+    const int m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const float* A = g.A + (size_t)b * g.sAb;
+    const float* B = g.B + (size_t)b * g.sBb;
+    const bool a_mfast = g.sAm == 1;
+    const bool b_nfast = g.sBn == 1;
+    const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+    v4f acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < g.K; k0 += GBK) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + 256 * q;
+            int mm, kk;
+            if (a_mfast) { mm = e % GBM; kk = e / GBM; } else { kk = e % GBK; mm = e / GBK; }
+            const int gm = m0 + mm, gk = k0 + kk;
+            As[kk][mm] = (gm < g.M && gk < g.K) ? A[(size_t)gm * g.sAm + (size_t)gk * g.sAk] : 0.f;
+            int nn;
+            if (b_nfast) { nn = e % GBN; kk = e / GBN; } else { kk = e % GBK; nn = e / GBK; }
+            const int gn = n0 + nn, gk2 = k0 + kk;
+            Bs[kk][nn] = (gn < g.N && gk2 < g.K) ? B[(size_t)gk2 * g.sBk + (size_t)gn * g.sBn] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GBK; kk += 4) {
+            const int kr = kk + (lane >> 4);
+            const float a0 = As[kr][wm + (lane & 15)];
+            const float a1 = As[kr][wm + 16 + (lane & 15)];
+            const float b0 = Bs[kr][wn + (lane & 15)];
+            const float b1 = Bs[kr][wn + 16 + (lane & 15)];
+            acc[0][0] = mfma16x16x4(a0, b0, acc[0][0]);
+            acc[0][1] = mfma16x16x4(a0, b1, acc[0][1]);
+            acc[1][0] = mfma16x16x4(a1, b0, acc[1][0]);
+            acc[1][1] = mfma16x16x4(a1, b1, acc[1][1]);
+        }
+        __syncthreads();
+    }
+    float* C = g.C + (size_t)b * g.sCb;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gm = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+                const int gn = n0 + wn + j * 16 + (lane & 15);
+                if (gm < g.M && gn < g.N) C[(size_t)gm * g.sCm + (size_t)gn * g.sCn] = g.alpha * acc[i][j][r];
+            }
+}
+
+// ---------------------------------------------------------------------------
+// Window gather: X[k][l][col] = D(shot, k, trace(col, l)), 0 beyond L(col).
+// D layout: D[shot][k2][g] with Re at k2 = k and Im at k2 = npair + k.
+// ---------------------------------------------------------------------------
+__global__ void radon_gather_kernel(const float* D, const DispCol* cols, int ncol, int npair, int ng, int Lmax,
+                                    float2* X)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    const int k = blockIdx.z;
+    if (col >= ncol) return;
+    const DispCol c = cols[col];
+    float2 v = make_float2(0.f, 0.f);
+    if (c.enabled && l < c.L) {
+        const int g = c.tr0 + c.dir * l;
+        const float* d = D + (size_t)c.shot * 2 * npair * ng;
+        v.x = d[(size_t)k * ng + g];
+        v.y = d[(size_t)(npair + k) * ng + g];
+    }
+    X[((size_t)k * Lmax + l) * ncol + col] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Complex slant stack with the Radon kernel generated in registers.
+// Forward (ADJ = false), for each frequency k:
+//   OUT[k][j][col] = sum_l exp(+i phi(j,l)) IN[k][l][col]     (j < np, l < Lmax)
+// Adjoint (ADJ = true):
+//   OUT[k][l][col] = sum_j exp(-i phi(j,l)) IN[k][j][col]     (l < Lmax, j < np)
+// phi(j,l) = 2 pi * frac(f_k * p_j * x_l), f_k = df (c0 + k), p_j = 1/(vmin + j),
+// x_l = (m + l) ds.  The phase is reduced in double, sin/cos taken in float.
+// 64 (rows) x 64 (cols) tile per block, 4 real MFMA 16x16x4 per complex step.
+// ---------------------------------------------------------------------------
+template <bool ADJ>
+__global__ __launch_bounds__(256) void slant_stack_kernel(SlantArgs a)
+{
+    __shared__ float Ar[GBK][GBM + 4], Ai[GBK][GBM + 4];
+    __shared__ float Br[GBK][GBN + 4], Bi[GBK][GBN + 4];
+    const int k = blockIdx.z;
+    const int m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int M = ADJ ? a.Lmax : a.np;      // output rows
+    const int K = ADJ ? a.np : a.Lmax;      // contraction length
+    const double fk = a.df * (double)(a.c0 + k);
+    const float2* IN = a.in + (size_t)k * K * a.ncol;
+    const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+    v4f cr[2][2], ci[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) { cr[i][j] = v4f{0.f, 0.f, 0.f, 0.f}; ci[i][j] = cr[i][j]; }
+    for (int k0 = 0; k0 < K; k0 += GBK) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + 256 * q;
+            // generated operand: row mm (output row), contraction kk
+            const int mm = e % GBM, kk = e / GBM;
+            const int gm = m0 + mm, gk = k0 + kk;
+            float sr = 0.f, si = 0.f;
+            if (gm < M && gk < K) {
+                const int j = ADJ ? gk : gm;
+                const int l = ADJ ? gm : gk;
+                const double p = 1.0 / (a.vmin + (double)j);
+                const double x = (double)(a.m + l) * a.ds;
+                double u = fk * (p * x);
+                u -= floor(u);
+                float s, c;
+                sincospif(2.0f * (float)u, &s, &c);
+                sr = c;
+                si = ADJ ? -s : s;
+            }
+            Ar[kk][mm] = sr;
+            Ai[kk][mm] = si;
+            // memory operand: contraction kk2, column nn (col fastest in memory)
+            const int nn = e % GBN, kk2 = e / GBN;
+            const int gn = n0 + nn, gk2 = k0 + kk2;
+            float2 v = make_float2(0.f, 0.f);
+            if (gn < a.ncol && gk2 < K) v = IN[(size_t)gk2 * a.ncol + gn];
+            Br[kk2][nn] = v.x;
+            Bi[kk2][nn] = v.y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GBK; kk += 4) {
+            const int kr = kk + (lane >> 4);
+            float ar[2], ai[2], br[2], bi[2];
+            for (int i = 0; i < 2; ++i) {
+                ar[i] = Ar[kr][wm + i * 16 + (lane & 15)];
+                ai[i] = Ai[kr][wm + i * 16 + (lane & 15)];
+                br[i] = Br[kr][wn + i * 16 + (lane & 15)];
+                bi[i] = Bi[kr][wn + i * 16 + (lane & 15)];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    cr[i][j] = mfma16x16x4(ar[i], br[j], cr[i][j]);
+                    cr[i][j] = mfma16x16x4(-ai[i], bi[j], cr[i][j]);
+                    ci[i][j] = mfma16x16x4(ar[i], bi[j], ci[i][j]);
+                    ci[i][j] = mfma16x16x4(ai[i], br[j], ci[i][j]);
+                }
+        }
+        __syncthreads();
+    }
+    float2* OUT = a.out + (size_t)k * M * a.ncol;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gm = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+                const int gn = n0 + wn + j * 16 + (lane & 15);
+                if (gm < M && gn < a.ncol) OUT[(size_t)gm * a.ncol + gn] = make_float2(cr[i][j][r], ci[i][j][r]);
+            }
+}
+
+// ---------------------------------------------------------------------------
+// |mm|, column maximum, normalised spectrum (RTr.m:28-32, RTrAD.m:27-32).
+// One block per (col, k); outputs ml[col][k][j], mlr[col][k][j].
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void radon_abs_kernel(const float2* MM, int np, int npair, int ncol, float* ml,
+                                                        float* mlr)
+{
+    const int col = blockIdx.x, k = blockIdx.y;
+    __shared__ float red[256];
+    float mx = 0.f;
+    bool nan = false;
+    for (int j = threadIdx.x; j < np; j += 256) {
+        const float2 v = MM[((size_t)k * np + j) * ncol + col];
+        const float a = sqrtf(v.x * v.x + v.y * v.y);
+        mlr[((size_t)col * npair + k) * np + j] = a;
+        if (a != a) nan = true;
+        else mx = fmaxf(mx, a);
+    }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const float cmax = red[0];
+    (void)nan;
+    for (int j = threadIdx.x; j < np; j += 256) {
+        const size_t o = ((size_t)col * npair + k) * np + j;
+        ml[o] = mlr[o] / cmax;  // 0/0 -> NaN for an all-zero column, as in MATLAB
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pickFDC1 (pickFDC1.m:19-131) + LHDispPick (LHDispPick.m:4-12), one
+// wavefront per column.  findpeaks(-y) local minima: a sample starting a run
+// of equal values whose previous sample is larger and whose run is followed
+// by a larger sample (MATLAB findpeaks defaults; endpoints never count).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float lds_or_nan(const float* v, int i, int n) { return (i >= 0 && i < n) ? v[i] : NAN; }
+
+// is position p (0-based within v[lo..hi]) a findpeaks location of -v ?
+__device__ bool is_local_min(const float* v, int lo, int hi, int p)
+{
+    const int i = lo + p;
+    const float y = v[i];
+    if (y != y) return false;
+    if (i == lo) return false;
+    const float yp = v[i - 1];
+    if (!(yp > y)) return false;  // previous (last of previous run) must be larger (NaN fails)
+    int e = i;
+    while (e + 1 <= hi && v[e + 1] == y) ++e;
+    if (e + 1 > hi) return false;
+    const float yn = v[e + 1];
+    return yn > y;
+}
+
+__global__ __launch_bounds__(64) void pick_kernel(const float* ml, const DispCol* cols, int np, int npair,
+                                                  int lp, float fg, float lfg, double vmin, const double* cr0,
+                                                  double* cr_pre, double* res, int* status)
+{
+    const int col = blockIdx.x;
+    const int lane = threadIdx.x;
+    const DispCol c = cols[col];
+    __shared__ int idx[kMaxPair], dvmax[kMaxPair], dvmin[kMaxPair], chz[kMaxPair];
+    __shared__ float qj[kMaxPair];
+    __shared__ double fidx[kMaxPair];
+    if (!c.enabled) return;
+    const float* base = ml + (size_t)col * npair * np;
+    const int a = np;
+
+    // argmax over v[lo..hi] (NaN ignored, first index of the maximum); 1-based result
+    auto argmax = [&](const float* v, int lo, int hi) -> int {
+        float best = -INFINITY;
+        int bi = 0x7fffffff;
+        for (int i = lo + lane; i <= hi; i += 64) {
+            const float x = v[i];
+            if (x == x && (x > best || (x == best && i < bi))) { best = x; bi = i; }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const float ob = __shfl_xor(best, off);
+            const int oi = __shfl_xor(bi, off);
+            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        }
+        return bi == 0x7fffffff ? 1 : bi - lo + 1;
+    };
+    // first local minimum at or after position 0 of v[lo..hi] (1-based location), 0 if none
+    auto first_min = [&](const float* v, int lo, int hi) -> int {
+        const int n = hi - lo + 1;
+        for (int p0 = 0; p0 < n; p0 += 64) {
+            const int p = p0 + lane;
+            const bool f = p < n && is_local_min(v, lo, hi, p);
+            const unsigned long long m = __ballot(f);
+            if (m) return p0 + __ffsll((long long)m);   // 1-based
+        }
+        return 0;
+    };
+    // last local minimum of v[lo..hi] (1-based location), 0 if none
+    auto last_min = [&](const float* v, int lo, int hi) -> int {
+        const int n = hi - lo + 1;
+        for (int p0 = ((n - 1) / 64) * 64; p0 >= 0; p0 -= 64) {
+            const int p = p0 + lane;
+            const bool f = p < n && is_local_min(v, lo, hi, p);
+            const unsigned long long m = __ballot(f);
+            if (m) return p0 + 64 - __clzll((long long)m);  // 1-based
+        }
+        return 0;
+    };
+
+    const int L1 = lp - 1;  // 0-based index of lp
+    if (lane == 0) for (int i = 0; i < npair; ++i) { chz[i] = 1; qj[i] = NAN; dvmax[i] = 0; dvmin[i] = 0; }
+    __syncthreads();
+    int cur = argmax(base + (size_t)L1 * np, 0, a - 1);
+    if (lane == 0) idx[L1] = cur;
+    __syncthreads();
+    // left of lp
+    for (int i = L1 - 1; i >= 0; --i) {
+        const float* v = base + (size_t)i * np;
+        const int ip = idx[i + 1];
+        if (ip == a) {
+            if (lane == 0) { idx[i] = a; dvmax[i] = dvmax[i + 1]; dvmin[i] = dvmin[i + 1]; }
+            __syncthreads();
+            continue;
+        }
+        int dmax, dmin;
+        if (ip > a - 3) dmax = a;
+        else { const int f = first_min(v, ip - 1, a - 1); dmax = f ? ip + f : a; }
+        if (ip < 3) dmin = 1;
+        else { const int f = last_min(v, 0, ip - 1); dmin = f ? f : 1; }
+        const float q = (float)(dmax - dmin);
+        const float qprev = qj[i + 1];
+        int newidx;
+        if (q > qprev * lfg) {
+            if (lane == 0) { chz[i] = 0; dvmax[i] = dvmax[i + 1]; dvmin[i] = dvmin[i + 1]; qj[i] = (float)(dvmax[i + 1] - dvmin[i + 1]); idx[i] = ip; }
+        } else {
+            newidx = dmin + argmax(v, dmin - 1, dmax - 1) - 1;
+            if (lane == 0) { idx[i] = newidx; dvmax[i] = dmax; dvmin[i] = dmin; qj[i] = q; }
+        }
+        __syncthreads();
+    }
+    // right of lp
+    for (int i = L1 + 1; i < npair; ++i) {
+        const float* v = base + (size_t)i * np;
+        const int ip = idx[i - 1];
+        int dmax, dmin;
+        if (ip > a - 3) dmax = a;
+        else { const int f = first_min(v, ip - 1, a - 1); dmax = f ? ip + f : a; }
+        if (ip < 3) dmin = 1;
+        else { const int f = last_min(v, 0, ip - 1); dmin = f ? f : 1; }
+        const float q = (float)(dmax - dmin);
+        const float qprev = qj[i - 1];
+        if (q > fg * qprev) {
+            if (lane == 0) { chz[i] = 0; dvmax[i] = dvmax[i - 1]; dvmin[i] = dvmin[i - 1]; qj[i] = (float)(dvmax[i - 1] - dvmin[i - 1]); idx[i] = ip; }
+        } else {
+            const int newidx = dmin + argmax(v, dmin - 1, dmax - 1) - 1;
+            if (lane == 0) { idx[i] = newidx; dvmax[i] = dmax; dvmin[i] = dmin; qj[i] = q; }
+        }
+        __syncthreads();
+    }
+    // interp1(que, idx(que), 1:b, 'linear') then nearest extrapolation
+    if (lane == 0) {
+        int nq = 0, qfirst = -1, qlast = -1;
+        for (int i = 0; i < npair; ++i)
+            if (chz[i]) { ++nq; if (qfirst < 0) qfirst = i; qlast = i; }
+        int st = 0;
+        if (nq < 2) st = kErrPick;
+        int prev = -1;
+        for (int i = 0; i < npair; ++i) {
+            if (chz[i]) { fidx[i] = (double)idx[i]; prev = i; continue; }
+            if (i < qfirst || i > qlast) continue;
+            int nxt = i + 1;
+            while (!chz[nxt]) ++nxt;
+            const double t = (double)(i - prev) / (double)(nxt - prev);
+            fidx[i] = (double)idx[prev] + t * (double)(idx[nxt] - idx[prev]);
+        }
+        for (int i = 0; i < npair; ++i) {
+            if (nq == 0) { fidx[i] = (double)idx[i]; continue; }
+            if (i < qfirst) fidx[i] = (double)idx[qfirst];
+            if (i > qlast) fidx[i] = (double)idx[qlast];
+        }
+        double acc = 0.0;
+        for (int i = 0; i < npair; ++i) {
+            const double v = fidx[i] + vmin;
+            cr_pre[(size_t)col * npair + i] = v;
+            const double d = v - cr0[(size_t)col * npair + i];
+            acc += d * d;
+        }
+        res[col] = acc / npair;
+        status[col] = st;
+    }
+}
+
+// LHDispPick method 2 (LHDispPick.m:8-9): column argmax (first index, NaN ignored) + vmin.
+__global__ __launch_bounds__(64) void argmax_pick_kernel(const float* ml, int np, int npair, double vmin,
+                                                         const double* cr0, double* cr_pre)
+{
+    const int k = blockIdx.x, lane = threadIdx.x;
+    const float* v = ml + (size_t)k * np;
+    float best = -INFINITY;
+    int bi = 0x7fffffff;
+    for (int i = lane; i < np; i += 64) {
+        const float x = v[i];
+        if (x == x && (x > best || (x == best && i < bi))) { best = x; bi = i; }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const float ob = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    }
+    if (lane == 0) cr_pre[k] = (double)(bi == 0x7fffffff ? 1 : bi + 1) + vmin;
+}
+
+// ---------------------------------------------------------------------------
+// Soft-argmax backward + phase (ADWDgrad_1.m:21-34), one block per (col, k):
+//   G = mlr^(1/40);  E = softmax(beta G) (log-sum-exp in double);
+//   kappa = sum_j j E_j;  g = cr_pre - cr0 (0 -> 1);
+//   gml_j = 2 beta g E_j (j - kappa);  GMM[k][j][col] = (mm/|mm|) gml_j (0 when |mm| = 0)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void softargmax_kernel(const float* mlr, const float2* MM, const DispCol* cols,
+                                                         int np, int npair, int ncol, double beta,
+                                                         const double* cr_pre, const double* cr0, float2* GMM)
+{
+    const int col = blockIdx.x, k = blockIdx.y;
+    const DispCol c = cols[col];
+    __shared__ double red[256];
+    __shared__ double red2[256];
+    const float* G = mlr + ((size_t)col * npair + k) * np;
+    const size_t ostride = (size_t)ncol;
+    float2* out = GMM + (size_t)k * np * ncol + col;
+    if (!c.enabled) {
+        for (int j = threadIdx.x; j < np; j += 256) out[j * ostride] = make_float2(0.f, 0.f);
+        return;
+    }
+    double mx = -INFINITY;
+    for (int j = threadIdx.x; j < np; j += 256) mx = fmax(mx, beta * pow((double)G[j], 1.0 / 40.0));
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double gmax = red[0];
+    __syncthreads();
+    double se = 0.0, sj = 0.0;
+    for (int j = threadIdx.x; j < np; j += 256) {
+        const double e = exp(beta * pow((double)G[j], 1.0 / 40.0) - gmax);
+        se += e;
+        sj += (double)(j + 1) * e;
+    }
+    red[threadIdx.x] = se;
+    red2[threadIdx.x] = sj;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) { red[threadIdx.x] += red[threadIdx.x + s]; red2[threadIdx.x] += red2[threadIdx.x + s]; }
+        __syncthreads();
+    }
+    const double Z = red[0];
+    const double kappa = red2[0] / Z;
+    double g = cr_pre[(size_t)col * npair + k] - cr0[(size_t)col * npair + k];
+    if (g == 0.0) g = 1.0;
+    for (int j = threadIdx.x; j < np; j += 256) {
+        const double E = exp(beta * pow((double)G[j], 1.0 / 40.0) - gmax) / Z;
+        const double gml = ((E * (double)(j + 1) - E * kappa) * 2.0 * g) * beta;
+        const float2 m = MM[((size_t)k * np + j) * ncol + col];
+        const double am = sqrt((double)m.x * m.x + (double)m.y * m.y);
+        float2 o = make_float2(0.f, 0.f);
+        if (am > 0.0) o = make_float2((float)((double)m.x / am * gml), (float)((double)m.y / am * gml));
+        out[j * ostride] = o;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Placement of the adjoint slant stack into the inverse-DFT operand:
+//   Bm[shot][g][k] = Re GD[k][l][col], Bm[shot][g][npair + k] = Im GD[k][l][col]
+// with g = trace(col, l); the left side skips l = 0 (ADWDgrad_1.m:76).
+// ---------------------------------------------------------------------------
+__global__ void adj_place_kernel(const float2* GD, const DispCol* cols, int ncol, int npair, int ng, int Lmax,
+                                 float* Bm)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y, col = blockIdx.z;
+    if (k >= npair) return;
+    const DispCol c = cols[col];
+    if (!c.enabled || l >= c.L || (c.dir < 0 && l == 0)) return;
+    const int g = c.tr0 + c.dir * l;
+    const float2 v = GD[((size_t)k * Lmax + l) * ncol + col];
+    float* o = Bm + ((size_t)c.shot * ng + g) * 2 * npair;
+    o[k] = v.x;
+    o[npair + k] = v.y;
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+int launch_dft_table(hipStream_t st, float* W, int nt_eff, int npair, int c0, int ccn)
+{
+    dft_table_kernel<<<dim3(div_up(npair, 128), nt_eff), 128, 0, st>>>(W, nt_eff, npair, c0, ccn);
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
+int launch_gemm_f32(hipStream_t st, const GemmArgs& g, int batch)
+{
+    if (g.M <= 0 || g.N <= 0 || batch <= 0) return kOk;
+    const dim3 grid(div_up(g.N, GBN), div_up(g.M, GBM), batch);
+    gemm_f32_kernel<<<grid, 256, 0, st>>>(g);
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
+int launch_radon_gather(hipStream_t st, const float* D, const DispCol* cols, int ncol, int npair, int ng, int Lmax,
+                        float2* X)
+{
+    radon_gather_kernel<<<dim3(div_up(ncol, 64), Lmax, npair), 64, 0, st>>>(D, cols, ncol, npair, ng, Lmax, X);
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
+int launch_slant_stack(hipStream_t st, const SlantArgs& a, bool adjoint)
+{
+    const int M = adjoint ? a.Lmax : a.np;
+    const dim3 grid(div_up(a.ncol, GBN), div_up(M, GBM), a.npair);
+    if (adjoint) slant_stack_kernel<true><<<grid, 256, 0, st>>>(a);
+    else slant_stack_kernel<false><<<grid, 256, 0, st>>>(a);
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
+int launch_radon_abs(hipStream_t st, const float2* MM, int np, int npair, int ncol, float* ml, float* mlr)
+{
+    radon_abs_kernel<<<dim3(ncol, npair), 256, 0, st>>>(MM, np, npair, ncol, ml, mlr);
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
+int launch_pick(hipStream_t st, const float* ml, const DispCol* cols, int ncol, int np, int npair, int lp, float fg,
+                float lfg, double vmin, const double* cr0, double* cr_pre, double* res, int* status)
+{
+    pick_kernel<<<ncol, 64, 0, st>>>(ml, cols, np, npair, lp, fg, lfg, vmin, cr0, cr_pre, res, status);
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
+int launch_argmax_pick(hipStream_t st, const float* ml, int np, int npair, double vmin, const double* cr0,
+                       double* cr_pre)
+{
+    argmax_pick_kernel<<<npair, 64, 0, st>>>(ml, np, npair, vmin, cr0, cr_pre);
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
+int launch_softargmax(hipStream_t st, const float* mlr, const float2* MM, const DispCol* cols, int np, int npair,
+                      int ncol, double beta, const double* cr_pre, const double* cr0, float2* GMM)
+{
+    softargmax_kernel<<<dim3(ncol, npair), 256, 0, st>>>(mlr, MM, cols, np, npair, ncol, beta, cr_pre, cr0, GMM);
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
+int launch_adj_place(hipStream_t st, const float2* GD, const DispCol* cols, int ncol, int npair, int ng, int Lmax,
+                     float* Bm)
+{
+    adj_place_kernel<<<dim3(div_up(npair, 64), Lmax, ncol), 64, 0, st>>>(GD, cols, ncol, npair, ng, Lmax, Bm);
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
+}  // namespace swd

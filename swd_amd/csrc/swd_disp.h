@@ -1,0 +1,54 @@
+// Dispersion-path types and launchers (Radon, picking, soft-argmax backward).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "swd_internal.h"
+
+namespace swd {
+
+constexpr int kMaxPair = 1024;  // frequency columns a pick kernel can hold in LDS
+
+// One (shot, side) spectrum column.
+struct DispCol {
+    int enabled;  // branch guard of SWD.m:175/181 and ADWDgrad_1.m:13/47
+    int shot;     // index into the batch's gathers
+    int tr0;      // 0-based trace of window position l = 0
+    int dir;      // +1 right window, -1 left window (fliplr)
+    int L;        // window length (dataLen)
+};
+
+struct GemmArgs {
+    const float* A;
+    const float* B;
+    float* C;
+    int M, N, K;
+    long long sAb, sAm, sAk;
+    long long sBb, sBk, sBn;
+    long long sCb, sCm, sCn;
+    float alpha;
+};
+
+struct SlantArgs {
+    const float2* in;
+    float2* out;
+    int np, Lmax, ncol, npair, c0, m;
+    double df, vmin, ds;
+};
+
+int launch_dft_table(hipStream_t st, float* W, int nt_eff, int npair, int c0, int ccn);
+int launch_gemm_f32(hipStream_t st, const GemmArgs& g, int batch);
+int launch_radon_gather(hipStream_t st, const float* D, const DispCol* cols, int ncol, int npair, int ng, int Lmax,
+                        float2* X);
+int launch_slant_stack(hipStream_t st, const SlantArgs& a, bool adjoint);
+int launch_radon_abs(hipStream_t st, const float2* MM, int np, int npair, int ncol, float* ml, float* mlr);
+int launch_pick(hipStream_t st, const float* ml, const DispCol* cols, int ncol, int np, int npair, int lp, float fg,
+                float lfg, double vmin, const double* cr0, double* cr_pre, double* res, int* status);
+int launch_argmax_pick(hipStream_t st, const float* ml, int np, int npair, double vmin, const double* cr0,
+                       double* cr_pre);
+int launch_softargmax(hipStream_t st, const float* mlr, const float2* MM, const DispCol* cols, int np, int npair,
+                      int ncol, double beta, const double* cr_pre, const double* cr0, float2* GMM);
+int launch_adj_place(hipStream_t st, const float2* GD, const DispCol* cols, int ncol, int npair, int ng, int Lmax,
+                     float* Bm);
+
+}  // namespace swd

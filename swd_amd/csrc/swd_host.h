@@ -1,0 +1,106 @@
+// Host-side helpers of libswd: RAII device buffers and streams, and the
+// declarations of the kernel launchers defined in the .hip files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "swd_internal.h"
+
+namespace swd {
+
+const char* last_error();
+int order_pt(int fd_order);
+int grid_pitch(int nxbc);
+int fd_tile_rows();
+void set_fd_tile_rows(int tz);
+int fd_force_plain();
+
+template <typename T>
+void pack_grid(const T* colmajor, int nzbc, int nxbc, int pitch, T* out);
+
+int prepare_model(const double* vp, const double* vs, int nz, int nx, int nbc, double dx, double dt, int fd_order,
+                  double* temp, double* ca, double* cl, double* cm, double* cm1, double* b, double* b1);
+int ricker(double f, double dt, int nt, double* w);
+
+// Device allocation owned by the host object (freed on destruction).
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int alloc(size_t n)
+    {
+        release();
+        if (n == 0) n = 256;
+        if (hipMalloc(&p, n) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            set_error("hipMalloc of %zu bytes failed", n);
+            return kErrNoMem;
+        }
+        bytes = n;
+        return kOk;
+    }
+    int upload(const void* h, size_t n, hipStream_t st)
+    {
+        int rc = alloc(n);
+        if (rc) return rc;
+        SWD_HIP(hipMemcpyAsync(p, h, n, hipMemcpyHostToDevice, st));
+        return kOk;
+    }
+    template <typename U>
+    U* as() const { return (U*)p; }
+};
+
+// A default-priority HIP stream owned by the host object.
+struct Stream {
+    hipStream_t s = nullptr;
+    int create()
+    {
+        SWD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        return kOk;
+    }
+    ~Stream()
+    {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    }
+};
+
+struct MarchPlan {
+    std::vector<BlockMap> bmap;
+    int nblocks = 0;
+};
+
+template <typename T>
+int build_march_plan(std::vector<FdJob<T>>& jobs, int tz, MarchPlan& plan);
+
+template <typename T>
+int launch_fwd_march(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step);
+template <typename T>
+int launch_adj_march(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step);
+template <typename T>
+int plain_forward_step(hipStream_t st, const FdJob<T>& J, T* const w[4], int it);
+template <typename T>
+int plain_adjoint_step(hipStream_t st, const FdJob<T>& J, int it);
+
+}  // namespace swd

@@ -1,0 +1,88 @@
+// Internal types shared by the SWD HIP kernels and the host library.
+// Not part of the C-ABI (include/swd_abi.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+namespace swd {
+
+// Status codes (mirrored in include/swd_abi.h).
+enum : int {
+    kOk = 0,
+    kErrArg = -2,
+    kErrCfl = -3,
+    kErrNoMem = -4,
+    kErrHip = -5,
+    kErrPick = -6,
+    kErrNoDevice = -7,
+    kErrBusy = -8,
+};
+
+void set_error(const char* fmt, ...);
+
+#define SWD_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ::swd::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),     \
+                             __FILE__, __LINE__);                                       \
+            return ::swd::kErrHip;                                                      \
+        }                                                                               \
+    } while (0)
+
+// One shot (forward) or one adjoint solve in a batched FD launch.
+//
+// Internal grid layout (all fields of one job share it): row-major, x fastest,
+// `pitch` elements per row, nzbc rows; row = depth index (0..nzbc-1),
+// col = distance index (0..nxbc-1) of the reference's padded grid.  The outer
+// ring (row 0, row nzbc-1, col 0, col nxbc-1) of every state array stays 0.
+// State is ping-ponged: kernels read cur[] and write nxt[].
+template <typename T>
+struct FdJob {
+    // state, [0] = read buffer of this step, [1] = write buffer (swapped by the host)
+    T* uu[2];
+    T* ww[2];
+    T* xx[2];
+    T* zz[2];
+    T* xz[2];
+    // model coefficients (read-only, shared by all shots of a model)
+    const T* temp;
+    const T* b;
+    const T* b1;
+    const T* ca;
+    const T* cl;
+    const T* cm1;
+    // forward: wavelet s[nt]; adjoint: adjoint source [nt][ng] (g fastest)
+    const T* src;
+    // forward receivers [nt][ng] (g fastest); trace_u may be null
+    T* trace_w;
+    T* trace_u;
+    // snapshots [nt/in_wf][nsf][nz][nx]; nsf = 3 (fux, bwz, fuz+bwx) or 4 (fux, fuz, bwx, bwz)
+    T* snap;
+    // adjoint images [3][nz][nx] (cl_img, cm_img, illum), accumulated in double
+    double* img;
+    int nzbc, nxbc, pitch, nz, nx, nbc, pt;
+    int sz, sx, stype;      // 0-based source cell and source type (1..5; 0 = none)
+    int gz, gx, dg, ng;     // 0-based receiver row / first column, spacing, count
+    int nt, in_wf;          // steps and snapshot interval
+    int start;              // launch step at which this job's step 0 runs
+    int nsf;                // snapshot fields (3 or 4); 0 = no snapshots
+    int tiles_x, tiles_z, tz;  // march tiling (fused kernels)
+    T dtx, dt;
+};
+
+// Block -> (job, first tile) map entry for fused launches.
+struct BlockMap {
+    int job;
+    int tile0;
+};
+
+constexpr int kMarchLanes = 64;          // one wavefront
+constexpr int kMarchOut = kMarchLanes - 2;  // output columns per wavefront (1-col halo each side)
+constexpr int kMarchWaves = 4;           // wavefronts per workgroup
+
+inline int div_up(int a, int b) { return (a + b - 1) / b; }
+
+}  // namespace swd
