@@ -1,0 +1,181 @@
+"""SWD hot-path benchmark on one MI355X.
+
+Workload (one "step"): one SWD gradient of the single-model production-shots
+configuration - 100 x 600 cells at 0.5 m with 30-cell borders (132 x 660
+padded), 60 Ricker 25 Hz shots 5 m apart, 600 receivers, dt = 8e-5 s,
+nt = 6000, snapshots every 5 steps - all 60 shots in one batch:
+forward FD + snapshots, RTrAD/RTlAD (151 frequencies x 701 velocities),
+LHDispPick, ADWDgrad_1 (beta = 10), adjoint FD + imaging, shot reduction and
+dk_vs.  Synthetic inputs: checkerboard "true" model for the observed picks
+(computed before timing), 300 m/s homogeneous current model.
+
+Prints one JSON line (the last line of stdout).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def fd_bytes_per_launch(prob, nshots, adjoint):
+    """Algorithmic HBM bytes of one FD time-step launch over the batch (FP32):
+    60 B per padded cell-step (5 state fields read + written, 5 coefficients
+    read) plus, averaged over the in_wf-step snapshot cycle, 16 B per interior
+    cell per snapshot step (forward: strain written; adjoint: strain read) and,
+    for the adjoint, the image read-modify-write (3 images x 2 x 8 B, FP64)."""
+    nzbc = prob["nz"] + 2 + prob["nbc"]
+    nxbc = prob["nx"] + 2 * prob["nbc"]
+    cells = nzbc * nxbc
+    inner = prob["nz"] * prob["nx"]
+    per = 60.0 * cells + 16.0 * inner / prob["in_wf"]
+    if adjoint:
+        per += 48.0 * inner / prob["in_wf"]
+    return per * nshots
+
+
+def cpu_baseline(prob, mod, seconds_hint=15.0):
+    """The FP32 statement-structure C port of the MEX time loops (oracle/),
+    one shot per thread on the host cores: forward + adjoint of `threads`
+    shots of the production grid with nt cut to keep the sample ~10-30 s."""
+    from oracle import swd_oracle as O
+    L = O.lib()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    nz, nx, nbc = prob["nz"], prob["nx"], prob["nbc"]
+    nzbc, nxbc = nz + 2 + nbc, nx + 2 * nbc
+    nt = 1000
+    c = O.prep(mod["vp"], mod["vs"], nbc, prob["dx"], prob["dt"], 22)
+    f32 = {k: np.asfortranarray(v, dtype=np.float32) for k, v in c.items()}
+    dtx = prob["dt"] / prob["dx"]
+    ivf = np.array([nt, nzbc, nxbc, dtx, prob["ng"], 3, nbc + 1, 3, nbc + 1, 1, 5, 22, prob["in_wf"], nz, nx, 3],
+                   dtype=np.float32)
+    ivb = np.array([nt, nzbc, nxbc, dtx, prob["ng"], nbc, 3, nbc + 1, 1, 22, prob["in_wf"], nz, nx, prob["dt"]],
+                   dtype=np.float32)
+    s = np.ascontiguousarray(mod["s"][:nt], dtype=np.float32)
+    sx = np.ascontiguousarray(prob["sx"][:threads] + nbc, dtype=np.int32)
+    nsh = sx.size
+    rng = np.random.default_rng(0)
+    adj = np.asfortranarray(rng.standard_normal((nsh * nt, prob["ng"])).astype(np.float32) * 1e-3)
+    sw = np.zeros(nsh * nt * prob["ng"], dtype=np.float32)
+    g = [np.zeros(nz * nx, dtype=np.float32) for _ in range(3)]
+    p = O._p
+    t0 = time.perf_counter()
+    failed = L.oracle_batch_f(p(ivf), p(ivb), *[p(f32[k]) for k in O.COEF_ORDER], p(s), p(sx), nsh, p(adj), p(sw),
+                              p(g[0]), p(g[1]), p(g[2]), threads)
+    wall = time.perf_counter() - t0
+    if failed:
+        raise RuntimeError("CPU baseline shots failed")
+    cell_steps = nsh * nzbc * nxbc * (nt + nt - 1)
+    return {"value": cell_steps / wall / 1e9, "unit": "Gcell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{nsh} shots (one per thread) of the 132x660 grid, nt={nt} of 6000, forward+adjoint, "
+                      f"FP32 C restatement of the MEX statement loops, {wall:.1f} s on {platform.processor() or 'host'}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--shots", type=int, default=60)
+    ap.add_argument("--nt", type=int, default=6000)
+    ap.add_argument("--tile-rows", type=int, default=0)
+    ap.add_argument("--plain-shots", type=int, default=2, help="shots for the unfused comparator (0: skip)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the shot-parallel multi-GPU "
+              "design is described in DESIGN.md but not built", file=sys.stderr)
+        sys.exit(2)
+
+    from swd_amd import problems as P
+    from swd_amd.engine import Engine
+
+    prob, mod = P.production_shots(nshots=args.shots, nt=args.nt)
+    eng = Engine(prob)
+    if args.tile_rows:
+        eng.set_tile_rows(args.tile_rows)
+    eng.set_wavelet(mod["s"])
+    eng.set_model(mod["vp_true"], mod["vs_true"])
+    eng.observe()                          # observed picks cr_0 / cr_0l (untimed, SWD.m:150-162)
+    eng.set_model(mod["vp"], mod["vs"])
+    for _ in range(args.warmup):
+        eng.gradient()
+    per_step, fwd_us, adj_us, stage = [], [], [], []
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        g = eng.gradient()                 # synchronous: returns after the last kernel and copy
+        per_step.append(time.perf_counter() - t0)
+        t = eng.timings()
+        fwd_us.append(t["fwd_kernel_us"])
+        adj_us.append(t["adj_kernel_us"])
+        stage.append(t)
+    wall = time.perf_counter() - t_all
+    info = eng.info()
+    S, nt = prob["sx"].size, prob["nt"]
+    cells = info["nzbc"] * info["nxbc"]
+    cell_steps = S * cells * (nt + nt - 1)
+    ms_step = 1000.0 * wall / args.steps
+    value = cell_steps / (wall / args.steps) / 1e9
+
+    # roofline of the dominant kernel (forward or adjoint march), event-timed in the run
+    f_us, a_us = float(np.mean(fwd_us)), float(np.mean(adj_us))
+    if f_us * nt >= a_us * (nt - 1):
+        kname, kus, kbytes = "fd_fwd_march", f_us, fd_bytes_per_launch(prob, S, False)
+    else:
+        kname, kus, kbytes = "fd_adj_march", a_us, fd_bytes_per_launch(prob, S, True)
+    achieved = kbytes / (kus * 1e-6) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": kname,
+            "kernel_us": round(kus, 2), "algorithmic_bytes_per_launch": int(kbytes)}
+
+    out = {"metric": "Gcell-steps/s FD fwd+adj (% HBM roofline), shots-grad/s; p50/p99 req latency",
+           "value": round(value, 3), "unit": "Gcell-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_step, 2), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (checkerboard true model, homogeneous current model, Ricker 25 Hz)",
+           "config": {"workload": "single-model production shots: 60 shots, 100x600 cells (132x660 padded), "
+                                  "nt=6000, dt=8e-5, 600 receivers; Radon 151 freq x 701 vel; beta=10",
+                      "shots": S, "nt": nt, "grid": [info["nzbc"], info["nxbc"]], "parallelism": "1 GPU, shot batch"},
+           "shots_grad_per_s": round(S / (wall / args.steps), 3),
+           "latency_ms": {"p50": round(1000 * float(np.percentile(per_step, 50)), 2),
+                          "p99": round(1000 * float(np.percentile(per_step, 99)), 2)},
+           "stage_ms": {k: round(float(np.mean([s[k] for s in stage])), 2)
+                        for k in ("fwd_ms", "disp_ms", "adj_ms", "asm_ms")},
+           "roofline": roof,
+           "device_bytes": info["device_bytes"],
+           "residual_r_mean": float(np.mean(g["res_r"]))}
+
+    if args.plain_shots > 0:
+        q2, m2 = P.production_shots(nshots=args.plain_shots, nt=args.nt)
+        e2 = Engine(q2)
+        e2.set_wavelet(m2["s"])
+        e2.set_observed_picks(np.ones((e2.npair, args.plain_shots)), np.ones((e2.npair, args.plain_shots)))
+        e2.set_model(m2["vp"], m2["vs"])
+        e2.set_fd_path(1)
+        t0 = time.perf_counter()
+        e2.gradient()
+        tp = time.perf_counter() - t0
+        plain = args.plain_shots * cells * (nt + nt - 1) / tp / 1e9
+        out["plain_path"] = {"value": round(plain, 3), "unit": "Gcell-steps/s",
+                             "what": f"unfused per-statement kernels, one shot per launch, {args.plain_shots} shots",
+                             "speedup": round(value / plain, 2)}
+        e2.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(prob, mod)
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -142,6 +142,64 @@ int swd_adwdgrad1(int nt, int ng, int ns, int npair, int is, double w, int m, in
                   int posr, int Lr, const float *mml, const float *mll, int posl, int Ll, int np, double vmin,
                   double df, double ds, int lf, int ccn, float *a_data_res);
 
+/* ------------------------------------------------------------------------
+ * Batched, device-resident engine: all shots of one SWD iteration of one
+ * model in HBM, one FD launch per time step for the whole batch.
+ * ------------------------------------------------------------------------ */
+
+typedef struct swd_engine swd_engine;
+
+/* One SWD problem (the constants of SWD.m:78-116 for one model).
+ * Positions are 1-based on the model grid, as SWD.m passes them to the
+ * wrappers: shot s at (sz[s], sx[s]); receiver g at (gz, gx0 + g*dg). */
+typedef struct swd_problem {
+    int nz, nx, nbc, fd_order, nt, in_wf;
+    double dx, dt;
+    int nshots;
+    const int *sx, *sz;
+    int ng, gx0, dg, gz;
+    double df, fmin, fmax, vmin, vmax, ds;  /* Radon axes; ds = receiver spacing in metres */
+    int offset, m, M, ini;                  /* windows (RTr.m:5-12), pick start column */
+    double w, beta;                         /* branch-guard w (SWD.m:112), SoftArgNorm */
+    const int *is_index;                    /* shot number used by windows and guards; NULL: s+1 */
+    int ns;                                 /* ns of the guards; 0: nshots */
+} swd_problem;
+
+/* The parfor body SWD.m:171-194 for every shot plus SWD.m:199-201:
+ * swd_engine_gradient runs forward FD (+ snapshots), RTrAD/RTlAD,
+ * LHDispPick, ADWDgrad_1, adjoint FD + imaging, the shot reduction and
+ * dk_vs = -4 vs g_cl + 2 vs g_cm (dk_vs nz x nx, before smooth2a).
+ * res_r/res_l: nshots; cr_r/cr_l: npair x nshots picks (any may be NULL).
+ * swd_engine_observe computes the observed picks cr_0 / cr_0l of
+ * SWD.m:150-162 from the current model and keeps them for the gradient;
+ * swd_engine_set_observed_picks installs given ones.  swd_engine_forward_picks
+ * is the forward-only line-search pass of SWD.m:215-228. */
+int swd_engine_create(const swd_problem *p, swd_engine **out);
+int swd_engine_destroy(swd_engine *e);
+int swd_engine_set_model(swd_engine *e, const double *vp, const double *vs);
+int swd_engine_set_wavelet(swd_engine *e, const double *s);
+int swd_engine_set_observed_picks(swd_engine *e, const double *cr0_r, const double *cr0_l);
+int swd_engine_observe(swd_engine *e, double *cr0_r, double *cr0_l);
+int swd_engine_gradient(swd_engine *e, double *dk_vs, double *res_r, double *res_l, double *cr_r, double *cr_l);
+int swd_engine_forward_picks(swd_engine *e, double *res_r, double *res_l, double *cr_r, double *cr_l);
+/* 0 = fused batched kernels (default); 1 = the plain per-statement kernels,
+ * one shot per launch (the unfused comparator).  Returns the previous mode. */
+int swd_engine_set_fd_path(swd_engine *e, int mode);
+int swd_engine_set_tile_rows(swd_engine *e, int tz);
+/* vals[0..9] = nzbc, nxbc, pitch, nt/in_wf, npair, np, device bytes, ccn, lf, tile rows */
+int swd_engine_info(swd_engine *e, long long *vals, int n);
+/* ms[0..5] = forward, dispersion, adjoint, assembly (ms) of the last gradient,
+ * mean forward / adjoint march-kernel duration (us, event-timed sample) */
+int swd_engine_timings(swd_engine *e, double *ms, int n);
+/* Copy a device buffer to host: 0 gathers [shot][t][g] f32, 1 adjoint sources
+ * [shot][t][g] f32, 2 {dk_vs, sum cl_img, sum cm_img, sum illum} nz x nx f64,
+ * 3 spectra [col][npair][np] f32, 4 per-shot images [shot][3][nz][nx] f64. */
+int swd_engine_read(swd_engine *e, int what, void *dst, long long bytes);
+/* Gradients of n engines (models of different sizes and nt) with shared FD
+ * launches; dk_vs[i] is nz_i x nx_i.  kernel_us[2] = sampled forward/adjoint
+ * march-kernel duration. */
+int swd_batch_gradient(swd_engine **engines, int n, double **dk_vs, double *kernel_us);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,20 @@ SIGNATURES = {
     "swd_lhdisppick": (_I, [_P, _I, _I, _D, _P, _I, _I, _P, _P]),
     "swd_adwdgrad1": (_I, [_I, _I, _I, _I, _I, _D, _I, _I, _D, _P, _P, _P, _P, _I, _I, _P, _P, _I, _I, _I, _D,
                            _D, _D, _I, _I, _P]),
+    "swd_engine_create": (_I, [_P, _P]),
+    "swd_engine_destroy": (_I, [_P]),
+    "swd_engine_set_model": (_I, [_P, _P, _P]),
+    "swd_engine_set_wavelet": (_I, [_P, _P]),
+    "swd_engine_set_observed_picks": (_I, [_P, _P, _P]),
+    "swd_engine_observe": (_I, [_P, _P, _P]),
+    "swd_engine_gradient": (_I, [_P, _P, _P, _P, _P, _P]),
+    "swd_engine_forward_picks": (_I, [_P, _P, _P, _P, _P]),
+    "swd_engine_set_fd_path": (_I, [_P, _I]),
+    "swd_engine_set_tile_rows": (_I, [_P, _I]),
+    "swd_engine_info": (_I, [_P, _P, _I]),
+    "swd_engine_timings": (_I, [_P, _P, _I]),
+    "swd_engine_read": (_I, [_P, _I, _P, ctypes.c_longlong]),
+    "swd_batch_gradient": (_I, [_P, _I, _P, _P]),
 }
 
 _lock = threading.Lock()
@@ -75,9 +89,11 @@ def check(rc: int) -> None:
 
 
 def ptr(a) -> ctypes.c_void_p:
+    """Data pointer of a numpy array; the returned object keeps the array alive
+    for as long as it exists (so temporaries survive until the C call returns)."""
     if a is None:
         return ctypes.c_void_p(None)
-    return ctypes.c_void_p(a.ctypes.data)
+    return a.ctypes.data_as(ctypes.c_void_p)
 
 
 def fortran(a, dtype) -> np.ndarray:

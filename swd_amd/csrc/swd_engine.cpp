@@ -1,0 +1,513 @@
+// Batched SWD engine (see swd_engine.h) and its C ABI.
+//
+// One gradient is the parfor body of SWD.m:171-194 for every shot, then the
+// reduction and assembly of SWD.m:193,199-201:
+//   forward FD (+ strain snapshots)       all shots, one launch per time step
+//   RTrAD/RTlAD -> LHDispPick            all (shot, side) columns batched
+//   ADWDgrad_1                           all columns batched
+//   adjoint FD + imaging                 all shots, one launch per time step
+//   image reduction + dk_vs              one kernel
+#include "swd_engine.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+namespace swd {
+
+Engine::~Engine()
+{
+    if (st) (void)hipStreamSynchronize(st);
+    if (own_stream && st) (void)hipStreamDestroy(st);
+}
+
+int Engine::create(const swd_problem* prob, hipStream_t stream)
+{
+    if (!prob) { set_error("null problem"); return kErrArg; }
+    p = *prob;
+    if (p.nz < 1 || p.nx < 1 || p.nbc < 2 || p.nt < 2 || p.in_wf < 1 || p.nt % p.in_wf != 0 || p.nshots < 1 ||
+        p.ng < 1 || p.dg < 1 || !(p.dx > 0) || !(p.dt > 0)) {
+        set_error("bad problem sizes (need nz,nx>=1, nbc>=2, nt%%in_wf==0, shots and receivers)");
+        return kErrArg;
+    }
+    if (p.fd_order != 22) { set_error("the batched engine runs fd_order 22 (the SWD configuration)"); return kErrArg; }
+    if (!p.sx || !p.sz) { set_error("null source positions"); return kErrArg; }
+    pt_m = (p.fd_order - 20) / 2 + 1;
+    nzbc = p.nz + pt_m + p.nbc;
+    nxbc = p.nx + 2 * p.nbc;
+    pitch = grid_pitch(nxbc);
+    plane = (size_t)nzbc * pitch;
+    nrec = p.nt / p.in_wf;
+    splane = (size_t)p.nz * p.nx;
+    sx.assign(p.sx, p.sx + p.nshots);
+    sz.assign(p.sz, p.sz + p.nshots);
+    is_index.resize(p.nshots);
+    for (int s = 0; s < p.nshots; ++s) is_index[s] = p.is_index ? p.is_index[s] : s + 1;
+    const int pt = 1;  // C-side pad_top of fd_order 22
+    for (int s = 0; s < p.nshots; ++s) {
+        const int r = sz[s] + pt_m - 1, c = sx[s] + p.nbc - 1;
+        if (r < pt || r > nzbc - 1 - pt || c < pt || c > nxbc - 1 - pt) {
+            set_error("shot %d lies outside the update block", s);
+            return kErrArg;
+        }
+    }
+    {
+        const int r = p.gz + pt_m - 1, c0 = p.gx0 + p.nbc - 1, c1 = c0 + (p.ng - 1) * p.dg;
+        if (r < pt || r > nzbc - 1 - pt || c0 < pt || c1 > nxbc - 1 - pt) {
+            set_error("receivers lie outside the update block");
+            return kErrArg;
+        }
+    }
+    if (stream) st = stream;
+    else {
+        SWD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        own_stream = true;
+    }
+    tz = fd_tile_rows();
+    // dispersion plan
+    DispGeom g;
+    memset(&g, 0, sizeof g);
+    g.nt = p.nt; g.ng = p.ng; g.nshots = p.nshots; g.df = p.df; g.dt = p.dt; g.fmin = p.fmin; g.fmax = p.fmax;
+    g.vmin = p.vmin; g.vmax = p.vmax; g.ds = p.ds; g.offset = p.offset; g.m = p.m; g.M = p.M;
+    int rc = disp_geometry(g);
+    if (rc) return rc;
+    if (p.M == 0) { set_error("M must be non-zero"); return kErrArg; }
+    npair = g.npair;
+    np = g.np;
+    if (p.ini < 1 || p.ini > npair) { set_error("ini %d outside 1..%d", p.ini, npair); return kErrArg; }
+    const int ns = p.ns > 0 ? p.ns : p.nshots;
+    auto mr = [](double x) { return (int)(x >= 0 ? std::floor(x + 0.5) : -std::floor(-x + 0.5)); };
+    std::vector<DispCol> cols(2 * p.nshots);
+    for (int side = 0; side < 2; ++side)
+        for (int s = 0; s < p.nshots; ++s) {
+            const int is = is_index[s];
+            DispCol c = window_col(side, is, p.ng, p.offset, p.m, p.M, s);
+            const bool guard = side == 0 ? is <= ns - (int)std::floor((double)p.m / p.M) - mr(p.w / p.M)
+                                         : is >= mr(p.w / p.M) + (int)std::floor((double)p.m / p.M) + 1;
+            if (!guard) c.enabled = 0;
+            cols[side * p.nshots + s] = c;
+        }
+    if ((rc = disp.init(g, cols, st))) return rc;
+    // device buffers
+    const size_t S = p.nshots, ntr = (size_t)p.nt * p.ng;
+    if ((rc = coefs.alloc(6 * plane * sizeof(float))) || (rc = vs_d.alloc(splane * sizeof(double))) ||
+        (rc = wavelet.alloc(p.nt * sizeof(float))) || (rc = state.alloc(S * 10 * plane * sizeof(float))) ||
+        (rc = traces.alloc(S * ntr * sizeof(float))) ||
+        (rc = snaps.alloc(S * (size_t)nrec * 3 * splane * sizeof(float))) ||
+        (rc = adjsrc.alloc(S * ntr * sizeof(float))) || (rc = imgs.alloc(S * 3 * splane * sizeof(double))) ||
+        (rc = outs.alloc(4 * splane * sizeof(double))))
+        return rc;
+    SWD_HIP(hipMemsetAsync(coefs.p, 0, coefs.bytes, st));
+    return kOk;
+}
+
+size_t Engine::footprint_bytes() const
+{
+    const DevBuf* b[] = {&coefs, &vs_d, &wavelet, &state, &strain, &traces, &snaps, &adjsrc, &imgs, &outs,
+                         &disp.W, &disp.D, &disp.X, &disp.MM, &disp.ml, &disp.mlr, &disp.GMM, &disp.GD,
+                         &disp.Bm, &disp.crpre, &disp.cr0, &disp.res, &disp.status};
+    size_t t = 0;
+    for (auto* x : b) t += x->bytes;
+    return t;
+}
+
+int Engine::set_model(const double* vp, const double* vs)
+{
+    if (!vp || !vs) { set_error("null model"); return kErrArg; }
+    const size_t n = (size_t)nzbc * nxbc;
+    std::vector<double> c(7 * n);
+    int rc = prepare_model(vp, vs, p.nz, p.nx, p.nbc, p.dx, p.dt, p.fd_order, c.data(), c.data() + n,
+                           c.data() + 2 * n, c.data() + 3 * n, c.data() + 4 * n, c.data() + 5 * n, c.data() + 6 * n);
+    if (rc) return rc;
+    vpmax = 0;
+    for (size_t k = 0; k < splane; ++k) vpmax = std::max(vpmax, std::max(vp[k], vs[k]));
+    // staggered 2nd-order stability bound in 2-D: vmax dt/dx <= 1/sqrt(2)
+    if (vpmax * p.dt / p.dx > 1.0 / std::sqrt(2.0)) {
+        set_error("CFL violated: vmax*dt/dx = %.4f > 0.7071", vpmax * p.dt / p.dx);
+        return kErrCfl;
+    }
+    // device order temp, b, b1, ca, cl, cm1 from prepare_model's temp, ca, cl, cm, cm1, b, b1
+    const int map[6] = {0, 5, 6, 1, 2, 4};
+    std::vector<float> h(6 * plane);
+    std::vector<float> tmp(n);
+    for (int k = 0; k < 6; ++k) {
+        const double* srcp = c.data() + (size_t)map[k] * n;
+        for (size_t q = 0; q < n; ++q) tmp[q] = (float)srcp[q];
+        pack_grid(tmp.data(), nzbc, nxbc, pitch, h.data() + (size_t)k * plane);
+    }
+    SWD_HIP(hipMemcpyAsync(coefs.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    SWD_HIP(hipMemcpyAsync(vs_d.p, vs, splane * sizeof(double), hipMemcpyHostToDevice, st));
+    SWD_HIP(hipStreamSynchronize(st));
+    have_model = true;
+    return kOk;
+}
+
+int Engine::set_wavelet(const double* s)
+{
+    if (!s) { set_error("null wavelet"); return kErrArg; }
+    std::vector<float> h(p.nt);
+    for (int k = 0; k < p.nt; ++k) h[k] = (float)s[k];
+    SWD_HIP(hipMemcpyAsync(wavelet.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    SWD_HIP(hipStreamSynchronize(st));
+    have_wavelet = true;
+    return kOk;
+}
+
+int Engine::set_observed_picks(const double* cr0_r, const double* cr0_l)
+{
+    const size_t n = (size_t)npair * p.nshots;
+    if (cr0_r) SWD_HIP(hipMemcpyAsync(disp.cr0.p, cr0_r, n * sizeof(double), hipMemcpyHostToDevice, st));
+    if (cr0_l)
+        SWD_HIP(hipMemcpyAsync((double*)disp.cr0.p + n, cr0_l, n * sizeof(double), hipMemcpyHostToDevice, st));
+    SWD_HIP(hipStreamSynchronize(st));
+    return kOk;
+}
+
+void Engine::make_jobs(std::vector<FdJob<float>>& jobs, bool adj, bool snapshots)
+{
+    float* C = coefs.as<float>();
+    for (int s = 0; s < p.nshots; ++s) {
+        FdJob<float> J;
+        memset(&J, 0, sizeof J);
+        float* S = state.as<float>() + (size_t)s * 10 * plane;
+        for (int b = 0; b < 2; ++b) {
+            J.uu[b] = S + (5 * b + 0) * plane;
+            J.ww[b] = S + (5 * b + 1) * plane;
+            J.xx[b] = S + (5 * b + 2) * plane;
+            J.zz[b] = S + (5 * b + 3) * plane;
+            J.xz[b] = S + (5 * b + 4) * plane;
+        }
+        J.temp = C; J.b = C + plane; J.b1 = C + 2 * plane; J.ca = C + 3 * plane; J.cl = C + 4 * plane;
+        J.cm1 = C + 5 * plane;
+        const size_t ntr = (size_t)p.nt * p.ng;
+        J.src = adj ? adjsrc.as<float>() + s * ntr : wavelet.as<float>();
+        J.trace_w = adj ? nullptr : traces.as<float>() + s * ntr;
+        J.trace_u = nullptr;
+        J.snap = snaps.as<float>() + (size_t)s * nrec * 3 * splane;
+        J.img = adj ? imgs.as<double>() + (size_t)s * 3 * splane : nullptr;
+        J.nzbc = nzbc; J.nxbc = nxbc; J.pitch = pitch; J.nz = p.nz; J.nx = p.nx; J.nbc = p.nbc; J.pt = 1;
+        J.sz = sz[s] + pt_m - 1; J.sx = sx[s] + p.nbc - 1; J.stype = adj ? 0 : 5;
+        J.gz = p.gz + pt_m - 1; J.gx = p.gx0 + p.nbc - 1; J.dg = p.dg; J.ng = p.ng;
+        J.nt = p.nt; J.in_wf = p.in_wf; J.start = 0; J.nsf = (adj || snapshots) ? 3 : 0;
+        J.dtx = (float)(p.dt / p.dx); J.dt = (float)p.dt;
+        jobs.push_back(J);
+    }
+}
+
+int run_fd_batch(const std::vector<Engine*>& engines, bool adj, bool snapshots, hipStream_t st, int tz,
+                 float* kernel_us)
+{
+    std::vector<FdJob<float>> jobs;
+    int steps = 0;
+    for (Engine* e : engines) {
+        e->make_jobs(jobs, adj, snapshots);
+        steps = std::max(steps, adj ? e->p.nt - 1 : e->p.nt);
+        SWD_HIP(hipMemsetAsync(e->state.p, 0, e->state.bytes, st));
+        if (adj) SWD_HIP(hipMemsetAsync(e->imgs.p, 0, e->imgs.bytes, st));
+        else SWD_HIP(hipMemsetAsync(e->traces.p, 0, e->traces.bytes, st));
+    }
+    MarchPlan plan;
+    int rc = build_march_plan(jobs, tz, plan);
+    if (rc) return rc;
+    DevBuf dj, db;
+    if ((rc = dj.upload(jobs.data(), jobs.size() * sizeof(FdJob<float>), st)) ||
+        (rc = db.upload(plan.bmap.data(), plan.bmap.size() * sizeof(BlockMap), st)))
+        return rc;
+    const FdJob<float>* J = dj.as<FdJob<float>>();
+    const BlockMap* B = db.as<BlockMap>();
+    // every 37th launch (a stride coprime to the snapshot interval) is bracketed
+    // by events on this stream: their mean is the per-launch kernel duration
+    const int stride = 37;
+    const int nsamp = (steps + stride - 1) / stride;
+    std::vector<hipEvent_t> ev(2 * nsamp);
+    for (auto& e : ev) SWD_HIP(hipEventCreate(&e));
+    for (int step = 0; step < steps; ++step) {
+        const bool samp = step % stride == 0;
+        const int si = step / stride;
+        if (samp) SWD_HIP(hipEventRecord(ev[2 * si], st));
+        rc = adj ? launch_adj_march<float>(st, J, B, plan.nblocks, step)
+                 : launch_fwd_march<float>(st, J, B, plan.nblocks, step);
+        if (rc) break;
+        if (samp) SWD_HIP(hipEventRecord(ev[2 * si + 1], st));
+    }
+    const hipError_t se = hipStreamSynchronize(st);
+    float tot = 0;
+    for (int i = 0; i < nsamp && rc == 0; ++i) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) == hipSuccess) tot += ms;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    if (se != hipSuccess) { set_error("FD batch failed: %s", hipGetErrorString(se)); return kErrHip; }
+    if (kernel_us) *kernel_us = nsamp > 0 ? 1000.f * tot / nsamp : 0.f;
+    return kOk;
+}
+
+int Engine::run_fd(bool adj, bool snapshots)
+{
+    if (fd_path == 0) {
+        float us = 0;
+        const int rc = run_fd_batch({this}, adj, snapshots, st, tz, &us);
+        if (adj) times.adj_kernel_us = us;
+        else times.fwd_kernel_us = us;
+        return rc;
+    }
+    // plain comparator: one shot at a time, one launch per reference statement
+    int rc;
+    if (!strain.p) {
+        if ((rc = strain.alloc(4 * plane * sizeof(float)))) return rc;
+    }
+    SWD_HIP(hipMemsetAsync(strain.p, 0, strain.bytes, st));
+    std::vector<FdJob<float>> jobs;
+    make_jobs(jobs, adj, snapshots);
+    if (adj) SWD_HIP(hipMemsetAsync(imgs.p, 0, imgs.bytes, st));
+    else SWD_HIP(hipMemsetAsync(traces.p, 0, traces.bytes, st));
+    float* w[4] = {strain.as<float>(), strain.as<float>() + plane, strain.as<float>() + 2 * plane,
+                   strain.as<float>() + 3 * plane};
+    for (auto& J : jobs) {
+        SWD_HIP(hipMemsetAsync(J.uu[0], 0, 10 * plane * sizeof(float), st));
+        if (adj) {
+            for (int it = p.nt - 2; it >= 0; --it)
+                if ((rc = plain_adjoint_step<float>(st, J, it))) return rc;
+        } else {
+            for (int it = 0; it < p.nt; ++it)
+                if ((rc = plain_forward_step<float>(st, J, w, it))) return rc;
+        }
+    }
+    SWD_HIP(hipStreamSynchronize(st));
+    return kOk;
+}
+
+int Engine::forward(bool snapshots)
+{
+    if (!have_model || !have_wavelet) { set_error("model and wavelet must be set first"); return kErrArg; }
+    return run_fd(false, snapshots);
+}
+int Engine::adjoint() { return run_fd(true, true); }
+
+int Engine::dispersion()
+{
+    int rc = disp.forward(traces.as<float>());
+    if (rc) return rc;
+    return disp.pick(p.ini, 2.f, 4.f);
+}
+
+int Engine::adjoint_source() { return disp.adjoint(p.beta, adjsrc.as<float>()); }
+
+int Engine::assemble()
+{
+    double* o = outs.as<double>();
+    return launch_assemble_gradient(st, imgs.as<double>(), p.nshots, p.nz, p.nx, vs_d.as<double>(), o, o + splane,
+                                    o + 2 * splane, o + 3 * splane);
+}
+
+int Engine::read_picks(double* res_r, double* res_l, double* cr_r, double* cr_l)
+{
+    const size_t S = p.nshots, n = (size_t)npair * S;
+    std::vector<int> stat(2 * S);
+    SWD_HIP(hipMemcpyAsync(stat.data(), disp.status.p, stat.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    if (res_r) SWD_HIP(hipMemcpyAsync(res_r, disp.res.p, S * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (res_l) SWD_HIP(hipMemcpyAsync(res_l, (double*)disp.res.p + S, S * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (cr_r) SWD_HIP(hipMemcpyAsync(cr_r, disp.crpre.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (cr_l) SWD_HIP(hipMemcpyAsync(cr_l, (double*)disp.crpre.p + n, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    SWD_HIP(hipStreamSynchronize(st));
+    for (size_t c = 0; c < 2 * S; ++c)
+        if (disp.cols[c].enabled && stat[c]) {
+            set_error("pick of shot %zu (%s side) kept fewer than two columns", c % S, c < S ? "right" : "left");
+            return kErrPick;
+        }
+    return kOk;
+}
+
+static float elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) ms = 0;
+    return ms;
+}
+
+int Engine::gradient(double* dk_vs, double* res_r, double* res_l, double* cr_r, double* cr_l)
+{
+    hipEvent_t e[5];
+    for (auto& x : e) SWD_HIP(hipEventCreate(&x));
+    int rc;
+    SWD_HIP(hipEventRecord(e[0], st));
+    if ((rc = forward(true))) goto done;
+    SWD_HIP(hipEventRecord(e[1], st));
+    if ((rc = dispersion()) || (rc = adjoint_source())) goto done;
+    SWD_HIP(hipEventRecord(e[2], st));
+    if ((rc = adjoint())) goto done;
+    SWD_HIP(hipEventRecord(e[3], st));
+    if ((rc = assemble())) goto done;
+    SWD_HIP(hipEventRecord(e[4], st));
+    if (dk_vs) SWD_HIP(hipMemcpyAsync(dk_vs, outs.p, splane * sizeof(double), hipMemcpyDeviceToHost, st));
+    rc = read_picks(res_r, res_l, cr_r, cr_l);
+    times.fwd_ms = elapsed(e[0], e[1]);
+    times.disp_ms = elapsed(e[1], e[2]);
+    times.adj_ms = elapsed(e[2], e[3]);
+    times.asm_ms = elapsed(e[3], e[4]);
+done:
+    for (auto& x : e) (void)hipEventDestroy(x);
+    return rc;
+}
+
+int Engine::forward_picks(double* res_r, double* res_l, double* cr_r, double* cr_l)
+{
+    int rc;
+    if ((rc = forward(false)) || (rc = dispersion())) return rc;
+    return read_picks(res_r, res_l, cr_r, cr_l);
+}
+
+int Engine::observe(double* cr0_r, double* cr0_l)
+{
+    int rc;
+    if ((rc = forward(false)) || (rc = dispersion())) return rc;
+    const size_t n = (size_t)npair * p.nshots;
+    SWD_HIP(hipMemcpyAsync(disp.cr0.p, disp.crpre.p, 2 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    return read_picks(nullptr, nullptr, cr0_r, cr0_l);
+}
+
+}  // namespace swd
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+using namespace swd;
+
+struct swd_engine {
+    Engine e;
+};
+
+extern "C" {
+
+int swd_engine_create(const swd_problem* p, swd_engine** out)
+{
+    if (!out) { set_error("null output handle"); return kErrArg; }
+    *out = nullptr;
+    swd_engine* h = new (std::nothrow) swd_engine;
+    if (!h) { set_error("out of host memory"); return kErrNoMem; }
+    const int rc = h->e.create(p, nullptr);
+    if (rc) { delete h; return rc; }
+    *out = h;
+    return kOk;
+}
+
+int swd_engine_destroy(swd_engine* e)
+{
+    delete e;
+    return kOk;
+}
+
+int swd_engine_set_model(swd_engine* e, const double* vp, const double* vs)
+{
+    return e ? e->e.set_model(vp, vs) : kErrArg;
+}
+
+int swd_engine_set_wavelet(swd_engine* e, const double* s) { return e ? e->e.set_wavelet(s) : kErrArg; }
+
+int swd_engine_set_observed_picks(swd_engine* e, const double* cr0_r, const double* cr0_l)
+{
+    return e ? e->e.set_observed_picks(cr0_r, cr0_l) : kErrArg;
+}
+
+int swd_engine_observe(swd_engine* e, double* cr0_r, double* cr0_l) { return e ? e->e.observe(cr0_r, cr0_l) : kErrArg; }
+
+int swd_engine_gradient(swd_engine* e, double* dk_vs, double* res_r, double* res_l, double* cr_r, double* cr_l)
+{
+    return e ? e->e.gradient(dk_vs, res_r, res_l, cr_r, cr_l) : kErrArg;
+}
+
+int swd_engine_forward_picks(swd_engine* e, double* res_r, double* res_l, double* cr_r, double* cr_l)
+{
+    return e ? e->e.forward_picks(res_r, res_l, cr_r, cr_l) : kErrArg;
+}
+
+int swd_engine_set_fd_path(swd_engine* e, int mode)
+{
+    if (!e) return kErrArg;
+    const int old = e->e.fd_path;
+    e->e.fd_path = mode ? 1 : 0;
+    return old;
+}
+
+int swd_engine_set_tile_rows(swd_engine* e, int tz)
+{
+    if (!e || tz < 1) { set_error("bad tile rows"); return kErrArg; }
+    e->e.tz = tz;
+    return kOk;
+}
+
+int swd_engine_info(swd_engine* e, long long* vals, int n)
+{
+    if (!e || !vals) return kErrArg;
+    const Engine& g = e->e;
+    const long long v[10] = {g.nzbc, g.nxbc, g.pitch, g.nrec, g.npair, g.np, (long long)g.footprint_bytes(),
+                             g.disp.g.ccn, g.disp.g.lf, g.tz};
+    for (int i = 0; i < n && i < 10; ++i) vals[i] = v[i];
+    return kOk;
+}
+
+int swd_engine_timings(swd_engine* e, double* ms, int n)
+{
+    if (!e || !ms) return kErrArg;
+    const StageTimes& t = e->e.times;
+    const double v[6] = {t.fwd_ms, t.disp_ms, t.adj_ms, t.asm_ms, t.fwd_kernel_us, t.adj_kernel_us};
+    for (int i = 0; i < n && i < 6; ++i) ms[i] = v[i];
+    return kOk;
+}
+
+int swd_engine_read(swd_engine* e, int what, void* dst, long long bytes)
+{
+    if (!e || !dst) return kErrArg;
+    Engine& g = e->e;
+    const DevBuf* src = nullptr;
+    size_t off = 0, n = 0;
+    switch (what) {
+    case 0: src = &g.traces; n = g.traces.bytes; break;                       // [shot][t][g] float
+    case 1: src = &g.adjsrc; n = g.adjsrc.bytes; break;                       // [shot][t][g] float
+    case 2: src = &g.outs; n = 4 * g.splane * sizeof(double); break;          // dk_vs, g_cl, g_cm, g_il
+    case 3: src = &g.disp.ml; n = g.disp.ml.bytes; break;                     // [col][npair][np] float
+    case 4: src = &g.imgs; n = g.imgs.bytes; break;                           // [shot][3][nz][nx] double
+    default: set_error("unknown buffer %d", what); return kErrArg;
+    }
+    if ((size_t)bytes < n) { set_error("destination too small (%lld < %zu)", bytes, n); return kErrArg; }
+    SWD_HIP(hipMemcpyAsync(dst, (const char*)src->p + off, n, hipMemcpyDeviceToHost, g.st));
+    SWD_HIP(hipStreamSynchronize(g.st));
+    return kOk;
+}
+
+int swd_batch_gradient(swd_engine** es, int n, double** dk_vs, double* kernel_us)
+{
+    if (!es || n < 1) { set_error("empty batch"); return kErrArg; }
+    std::vector<Engine*> v;
+    for (int i = 0; i < n; ++i) {
+        if (!es[i]) { set_error("null engine %d", i); return kErrArg; }
+        if (!es[i]->e.have_model || !es[i]->e.have_wavelet) { set_error("engine %d lacks model/wavelet", i); return kErrArg; }
+        v.push_back(&es[i]->e);
+    }
+    hipStream_t st = v[0]->st;
+    // every engine's stream must be idle before the shared launches
+    for (Engine* x : v) SWD_HIP(hipStreamSynchronize(x->st));
+    float us[2] = {0, 0};
+    int rc = run_fd_batch(v, false, true, st, v[0]->tz, &us[0]);
+    if (rc) return rc;
+    for (Engine* x : v) {
+        if ((rc = x->dispersion()) || (rc = x->adjoint_source())) return rc;
+        SWD_HIP(hipStreamSynchronize(x->st));
+    }
+    if ((rc = run_fd_batch(v, true, true, st, v[0]->tz, &us[1]))) return rc;
+    for (int i = 0; i < n; ++i) {
+        Engine* x = v[i];
+        if ((rc = x->assemble())) return rc;
+        if (dk_vs && dk_vs[i])
+            SWD_HIP(hipMemcpyAsync(dk_vs[i], x->outs.p, x->splane * sizeof(double), hipMemcpyDeviceToHost, x->st));
+        SWD_HIP(hipStreamSynchronize(x->st));
+    }
+    if (kernel_us) { kernel_us[0] = us[0]; kernel_us[1] = us[1]; }
+    return kOk;
+}
+
+}  // extern "C"

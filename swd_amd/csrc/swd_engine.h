@@ -1,0 +1,83 @@
+// The batched, device-resident SWD engine: one model, all shots of an
+// inversion iteration resident in HBM (state, gathers, strain snapshots,
+// spectra, adjoint sources, images).  Also the unit of work of the
+// multi-model batch runner and of the serving loop.
+#pragma once
+
+#include <vector>
+
+#include "swd_abi.h"
+#include "swd_disp_plan.h"
+#include "swd_host.h"
+
+namespace swd {
+
+int launch_assemble_gradient(hipStream_t st, const double* imgs, int nshots, int nz, int nx, const double* vs,
+                             double* dk, double* gcl, double* gcm, double* gil);
+
+struct StageTimes {
+    float fwd_ms = 0, disp_ms = 0, adj_ms = 0, asm_ms = 0;
+    float fwd_kernel_us = 0, adj_kernel_us = 0;  // mean duration of one march launch (sampled)
+    int fwd_samples = 0, adj_samples = 0;
+};
+
+class Engine {
+public:
+    swd_problem p{};
+    std::vector<int> sx, sz, is_index;  // copies of the problem arrays
+    int pt_m = 0, nzbc = 0, nxbc = 0, pitch = 0, nrec = 0, npair = 0, np = 0;
+    size_t plane = 0, splane = 0;
+    hipStream_t st = nullptr;
+    bool own_stream = false;
+    int fd_path = 0;  // 0 fused batched, 1 plain one-shot-per-launch
+    int tz = 16;
+    bool have_model = false, have_wavelet = false;
+    double vpmax = 0;
+
+    DevBuf coefs;     // 6 planes float: temp, b, b1, ca, cl, cm1
+    DevBuf vs_d;      // nz x nx double (MATLAB layout) for the assembly
+    DevBuf wavelet;   // nt float
+    DevBuf state;     // nshots x 10 planes float
+    DevBuf strain;    // 4 planes (plain path scratch)
+    DevBuf traces;    // [shot][t][g] float
+    DevBuf snaps;     // [shot][nrec][3][nz][nx] float
+    DevBuf adjsrc;    // [shot][t][g] float
+    DevBuf imgs;      // [shot][3][nz][nx] double
+    DevBuf outs;      // dk_vs, g_cl, g_cm, g_il (nz x nx double each)
+    DevBuf djobs, dbmap;
+    DispPlan disp;
+    StageTimes times;
+    int timing_samples = 8;
+
+    ~Engine();
+    int create(const swd_problem* prob, hipStream_t stream);
+    int set_model(const double* vp, const double* vs);
+    int set_wavelet(const double* s);
+    int set_observed_picks(const double* cr0_r, const double* cr0_l);
+    size_t footprint_bytes() const;
+
+    // building blocks (all enqueue on st; nothing synchronises unless stated)
+    void make_jobs(std::vector<FdJob<float>>& jobs, bool adjoint, bool snapshots);
+    int forward(bool snapshots);
+    int dispersion();
+    int adjoint_source();
+    int adjoint();
+    int assemble();
+
+    // complete operations (synchronous)
+    int gradient(double* dk_vs, double* res_r, double* res_l, double* cr_r, double* cr_l);
+    int forward_picks(double* res_r, double* res_l, double* cr_r, double* cr_l);
+    int observe(double* cr0_r, double* cr0_l);
+    int read_picks(double* res_r, double* res_l, double* cr_r, double* cr_l);
+
+private:
+    int run_fd(bool adjoint, bool snapshots);
+};
+
+// Run the FD phase of several engines in shared launches (one launch per
+// time step covers every shot of every engine; engines may differ in grid
+// size and nt).  adjoint selects the reverse-time phase.
+int run_fd_batch(const std::vector<Engine*>& engines, bool adjoint, bool snapshots, hipStream_t st, int tz,
+                 float* kernel_us);
+
+}  // namespace swd

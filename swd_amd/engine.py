@@ -1,0 +1,141 @@
+"""Python handle of the batched device-resident SWD engine (swd_engine_* in
+include/swd_abi.h).  One Engine = one model with all the shots of an SWD
+iteration resident in HBM; ``gradient()`` is the parfor body of
+SWD.m:171-194 for every shot plus the assembly of SWD.m:199-201."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._lib import check, fortran, load, ptr
+
+_I, _D, _P = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [("nz", _I), ("nx", _I), ("nbc", _I), ("fd_order", _I), ("nt", _I), ("in_wf", _I),
+                ("dx", _D), ("dt", _D), ("nshots", _I), ("sx", _P), ("sz", _P),
+                ("ng", _I), ("gx0", _I), ("dg", _I), ("gz", _I),
+                ("df", _D), ("fmin", _D), ("fmax", _D), ("vmin", _D), ("vmax", _D), ("ds", _D),
+                ("offset", _I), ("m", _I), ("M", _I), ("ini", _I), ("w", _D), ("beta", _D),
+                ("is_index", _P), ("ns", _I)]
+
+
+PROBLEM_KEYS = ("nz", "nx", "nbc", "fd_order", "nt", "in_wf", "dx", "dt", "ng", "gx0", "dg", "gz", "df", "fmin",
+                "fmax", "vmin", "vmax", "ds", "offset", "m", "M", "ini", "w", "beta", "ns")
+
+def lib():
+    return load()
+
+
+class Engine:
+    """Batched SWD engine for one model (FP32 FD + Radon, FP64 reductions)."""
+
+    def __init__(self, problem: dict):
+        self.problem = dict(problem)
+        q = self.problem
+        self._sx = np.ascontiguousarray(q["sx"], dtype=np.int32)
+        self._sz = np.ascontiguousarray(q["sz"], dtype=np.int32)
+        self._is = None if q.get("is_index") is None else np.ascontiguousarray(q["is_index"], dtype=np.int32)
+        p = Problem()
+        for k in PROBLEM_KEYS:
+            setattr(p, k, q.get(k, 0) if k != "fd_order" else q.get(k, 22))
+        p.nshots = int(self._sx.size)
+        p.sx = self._sx.ctypes.data
+        p.sz = self._sz.ctypes.data
+        p.is_index = None if self._is is None else self._is.ctypes.data
+        self._p = p
+        h = ctypes.c_void_p()
+        check(lib().swd_engine_create(ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+        info = self.info()
+        self.npair = int(info["npair"])
+        self.nshots = p.nshots
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().swd_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        v = np.zeros(10, dtype=np.int64)
+        check(lib().swd_engine_info(self.h, ptr(v), 10))
+        keys = ("nzbc", "nxbc", "pitch", "nrec", "npair", "np", "device_bytes", "ccn", "lf", "tile_rows")
+        return dict(zip(keys, (int(x) for x in v)))
+
+    def set_model(self, vp, vs):
+        self._vs = fortran(vs, np.float64)
+        check(lib().swd_engine_set_model(self.h, ptr(fortran(vp, np.float64)), ptr(self._vs)))
+
+    def set_wavelet(self, s):
+        check(lib().swd_engine_set_wavelet(self.h, ptr(np.ascontiguousarray(s, dtype=np.float64))))
+
+    def set_observed_picks(self, cr0_r, cr0_l):
+        a = fortran(cr0_r, np.float64)
+        b = fortran(cr0_l, np.float64)
+        check(lib().swd_engine_set_observed_picks(self.h, ptr(a), ptr(b)))
+
+    def observe(self):
+        a = np.zeros((self.npair, self.nshots), order="F")
+        b = np.zeros((self.npair, self.nshots), order="F")
+        check(lib().swd_engine_observe(self.h, ptr(a), ptr(b)))
+        return a, b
+
+    def gradient(self) -> dict:
+        q = self.problem
+        dk = np.zeros((q["nz"], q["nx"]), order="F")
+        rr, rl = np.zeros(self.nshots), np.zeros(self.nshots)
+        cr = np.zeros((self.npair, self.nshots), order="F")
+        cl = np.zeros((self.npair, self.nshots), order="F")
+        check(lib().swd_engine_gradient(self.h, ptr(dk), ptr(rr), ptr(rl), ptr(cr), ptr(cl)))
+        return {"dk_vs": dk, "res_r": rr, "res_l": rl, "cr_r": cr, "cr_l": cl}
+
+    def forward_picks(self) -> dict:
+        rr, rl = np.zeros(self.nshots), np.zeros(self.nshots)
+        cr = np.zeros((self.npair, self.nshots), order="F")
+        cl = np.zeros((self.npair, self.nshots), order="F")
+        check(lib().swd_engine_forward_picks(self.h, ptr(rr), ptr(rl), ptr(cr), ptr(cl)))
+        return {"res_r": rr, "res_l": rl, "cr_r": cr, "cr_l": cl}
+
+    def set_fd_path(self, mode: int) -> int:
+        return lib().swd_engine_set_fd_path(self.h, int(mode))
+
+    def set_tile_rows(self, tz: int):
+        check(lib().swd_engine_set_tile_rows(self.h, int(tz)))
+
+    def timings(self) -> dict:
+        v = np.zeros(6)
+        check(lib().swd_engine_timings(self.h, ptr(v), 6))
+        return dict(zip(("fwd_ms", "disp_ms", "adj_ms", "asm_ms", "fwd_kernel_us", "adj_kernel_us"), v))
+
+    def read(self, what: str) -> np.ndarray:
+        q = self.problem
+        S, nt, ng, nz, nx = self.nshots, q["nt"], q["ng"], q["nz"], q["nx"]
+        info = self.info()
+        shapes = {"gathers": (0, np.float32, (S, nt, ng)), "adjoint_sources": (1, np.float32, (S, nt, ng)),
+                  "gradient": (2, np.float64, (4, nx, nz)), "spectra": (3, np.float32, (2 * S, self.npair, info["np"])),
+                  "images": (4, np.float64, (S, 3, nz, nx))}
+        code, dt, shape = shapes[what]
+        out = np.zeros(shape, dtype=dt)
+        check(lib().swd_engine_read(self.h, code, ptr(out), out.nbytes))
+        if what == "gradient":
+            return np.transpose(out, (0, 2, 1))     # -> [4][nz][nx]
+        return out
+
+
+def batch_gradient(engines) -> tuple:
+    """Gradients of several engines with shared FD launches (uneven models)."""
+    n = len(engines)
+    arr = (ctypes.c_void_p * n)(*[e.h.value for e in engines])
+    outs = [np.zeros((e.problem["nz"], e.problem["nx"]), order="F") for e in engines]
+    parr = (ctypes.c_void_p * n)(*[o.ctypes.data for o in outs])
+    us = np.zeros(2)
+    check(lib().swd_batch_gradient(arr, n, parr, ptr(us)))
+    return outs, us

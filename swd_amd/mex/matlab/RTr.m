@@ -1,0 +1,4 @@
+function [ml,dataLen]=RTr(seismo_v1,is,df,dt,np,vmin,vmax,fmin,fmax,a,b,ds,offset,m,M)
+% Drop-in for core/RTr.m: linear Radon transform on the MI355X (libswd).
+[ml,dataLen]=swd_rt_mex(0,seismo_v1,is,df,dt,np,vmin,vmax,fmin,fmax,a,b,ds,offset,m,M);
+end

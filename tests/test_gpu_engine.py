@@ -1,0 +1,160 @@
+"""GPU parity of the batched engine (full SWD shot-gradient) against the FP64
+oracle, plus the batching invariants (a shot's result is the same alone and in
+a batch of other models).  Gradient tolerance: relative L2 <= 1e-4 when the
+FP32 picks equal the FP64 picks; picks within one velocity bin."""
+import numpy as np
+import pytest
+
+from oracle import swd_oracle as O
+from swd_amd import problems as P
+from swd_amd.engine import Engine, batch_gradient
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    """Relative L2 over finite entries; unilluminated cells are 0/0 in both
+    (g_cl./g_illum, SWD.m:199) and must be NaN in the same places."""
+    a, b = np.ravel(a), np.ravel(b)
+    assert np.array_equal(np.isnan(a), np.isnan(b))
+    m = np.isfinite(b)
+    return float(np.linalg.norm(a[m] - b[m]) / max(np.linalg.norm(b[m]), 1e-300))
+
+
+def oracle_observed(prob, mod, is_list):
+    """FP64 observed picks (SWD.m:150-162) from the true model."""
+    q = prob
+    gx = q["gx0"] + q["dg"] * np.arange(q["ng"], dtype=float)
+    gz = np.full(q["ng"], float(q["gz"]))
+    radon = (q["df"], q["dt"], int(q["vmax"] - q["vmin"] + 1), q["vmin"], q["vmax"], q["fmin"], q["fmax"], q["nt"],
+             q["ng"], q["ds"], q["offset"], q["m"], q["M"])
+    crr, crl = [], []
+    for k, is_ in enumerate(is_list):
+        _, sw, _ = O.staggerfd_eigend(is_, q["nbc"], q["nt"], q["dt"] / q["dx"], q["dx"], q["dt"], q["sx"][k],
+                                      q["sz"][k], gx, gz, mod["s"], mod["vp_true"], mod["vs_true"], 1, 22, "w", 0,
+                                      q["in_wf"], snapshots=False)
+        ml, _, _ = O.rt_side(sw, "r", is_, *radon)
+        crr.append(O.pickFDC1(ml, q["ini"], 2, 4) + q["vmin"])
+        if is_ > 1 or q["M"] == 1:
+            ml, _, _ = O.rt_side(sw, "l", is_, *radon)
+            crl.append(O.pickFDC1(ml, q["ini"], 2, 4) + q["vmin"])
+        else:
+            crl.append(np.ones_like(crr[-1]))
+    return np.array(crr).T, np.array(crl).T
+
+
+def oracle_gradient(prob, mod, is_list, cr0_r, cr0_l):
+    q = prob
+    gx = q["gx0"] + q["dg"] * np.arange(q["ng"], dtype=float)
+    gz = np.full(q["ng"], float(q["gz"]))
+    radon = dict(df=q["df"], fmin=q["fmin"], fmax=q["fmax"], vmin=q["vmin"], vmax=q["vmax"], ds=q["ds"],
+                 offset=q["offset"], m=q["m"], M=q["M"])
+    ns = q["ns"] if q["ns"] else len(is_list)
+    tot = None
+    shots = []
+    for k, is_ in enumerate(is_list):
+        o = O.swd_shot(mod["vp"], mod["vs"], mod["s"], q["nbc"], q["nt"], q["dx"], q["dt"], q["sx"][k], q["sz"][k],
+                       gx, gz, q["in_wf"], 22, is_, ns, radon, cr0_r[:, k], cr0_l[:, k], q["beta"], q["ini"], q["w"])
+        shots.append(o)
+        imgs = np.array([o["cl_img"], o["cm_img"], o["illum"]])
+        tot = imgs if tot is None else tot + imgs
+    return O.assemble_dk_vs(mod["vs"], *tot), shots, tot
+
+
+@pytest.fixture(scope="module")
+def single():
+    prob, mod = P.plain_single_shot(nt=1500)
+    cr0_r, cr0_l = oracle_observed(prob, mod, [5])
+    dk, shots, _ = oracle_gradient(prob, mod, [5], cr0_r, cr0_l)
+    return prob, mod, cr0_r, cr0_l, dk, shots
+
+
+def _engine(prob, mod, cr0_r, cr0_l):
+    e = Engine(prob)
+    e.set_model(mod["vp"], mod["vs"])
+    e.set_wavelet(mod["s"])
+    e.set_observed_picks(cr0_r, cr0_l)
+    return e
+
+
+def test_engine_single_shot_gradient_vs_fp64_oracle(single):
+    prob, mod, cr0_r, cr0_l, dk_ref, shots = single
+    e = _engine(prob, mod, cr0_r, cr0_l)
+    g = e.gradient()
+    o = shots[0]
+    gath = e.read("gathers")[0]
+    err = {"gather": rel(gath, o["seismo_w"])}
+    dr = np.max(np.abs(g["cr_r"][:, 0] - o["cr_r"]))
+    dl = np.max(np.abs(g["cr_l"][:, 0] - o["cr_l"]))
+    assert dr <= 1 and dl <= 1, (dr, dl)
+    if dr == 0 and dl == 0:
+        err["adjoint_source"] = rel(e.read("adjoint_sources")[0], o["adj"])
+        err["dk_vs"] = rel(g["dk_vs"], dk_ref)
+        assert g["res_r"][0] == pytest.approx(o["res_r"], rel=1e-9)
+    print("engine fp32 rel-L2 vs fp64 oracle:", err, "timings:", e.timings())
+    assert max(err.values()) < 1e-4, err
+    e.close()
+
+
+def test_engine_three_shots_vs_oracle():
+    nz, nx = 30, 120
+    vs = np.tile(np.linspace(220, 420, nz)[:, None], (1, nx))
+    vs_true = vs * (1 + 0.08 * P.checkerboard(nz, nx, 15, 1.0, 1.0) / 2)
+    dt, nt = 1e-4, 1500
+    prob = dict(nz=nz, nx=nx, nbc=10, fd_order=22, nt=nt, in_wf=5, dx=0.5, dt=dt, sx=np.array([1, 41, 81]),
+                sz=np.array([1, 1, 1]), ng=120, gx0=1, dg=1, gz=1, df=0.5, fmin=5.0, fmax=80.0, vmin=100.0,
+                vmax=800.0, ds=0.5, offset=120, m=0, M=40, ini=10, w=3.0, beta=10.0, ns=0)
+    mod = {"vs": vs, "vp": 1.73 * vs, "vs_true": vs_true, "vp_true": 1.73 * vs_true, "s": O.ricker(30, dt, nt)}
+    cr0_r, cr0_l = oracle_observed(prob, mod, [1, 2, 3])
+    dk_ref, shots, tot = oracle_gradient(prob, mod, [1, 2, 3], cr0_r, cr0_l)
+    e = _engine(prob, mod, cr0_r, cr0_l)
+    g = e.gradient()
+    same = True
+    for k, o in enumerate(shots):
+        assert np.max(np.abs(g["cr_r"][:, k] - o["cr_r"])) <= 1
+        same &= np.array_equal(g["cr_r"][:, k], o["cr_r"])
+        if k > 0:
+            assert np.max(np.abs(g["cr_l"][:, k] - o["cr_l"])) <= 1
+            same &= np.array_equal(g["cr_l"][:, k], o["cr_l"])
+    gath = e.read("gathers")
+    for k, o in enumerate(shots):
+        assert rel(gath[k], o["seismo_w"]) < 1e-4
+    if same:
+        err = rel(g["dk_vs"], dk_ref)
+        print("3-shot dk_vs rel-L2:", err)
+        assert err < 1e-4
+    e.close()
+
+
+def test_batch_is_the_same_alone_and_in_mix(single):
+    prob, mod, cr0_r, cr0_l, _, _ = single
+    a = _engine(prob, mod, cr0_r, cr0_l)
+    ga = a.gradient()["dk_vs"]
+    # a second, differently sized model in the same launches
+    nz, nx = 24, 90
+    vs = np.tile(np.linspace(250, 400, nz)[:, None], (1, nx))
+    prob2 = dict(nz=nz, nx=nx, nbc=12, fd_order=22, nt=1000, in_wf=5, dx=0.5, dt=1e-4, sx=np.array([1, 46]),
+                 sz=np.array([1, 1]), ng=90, gx0=1, dg=1, gz=1, df=0.5, fmin=5.0, fmax=80.0, vmin=100.0, vmax=800.0,
+                 ds=0.5, offset=90, m=0, M=45, ini=10, w=3.0, beta=10.0, ns=0)
+    b = Engine(prob2)
+    b.set_model(1.73 * vs, vs)
+    b.set_wavelet(O.ricker(30, 1e-4, 1000))
+    b.observe()
+    b.set_model(1.73 * vs * 1.02, vs * 1.02)
+    gb = b.gradient()["dk_vs"]
+    (ma, mb), us = batch_gradient([a, b])
+    np.testing.assert_array_equal(ma, ga)
+    np.testing.assert_array_equal(mb, gb)
+    a.close()
+    b.close()
+
+
+def test_plain_engine_path_matches_fused(single):
+    prob, mod, cr0_r, cr0_l, _, _ = single
+    e = _engine(prob, mod, cr0_r, cr0_l)
+    gf = e.gradient()
+    e.set_fd_path(1)
+    gp = e.gradient()
+    assert np.array_equal(gf["cr_r"], gp["cr_r"])
+    assert rel(gp["dk_vs"], gf["dk_vs"]) < 1e-5
+    e.close()
