@@ -79,6 +79,11 @@ int swd_e2drtm_f64(const double *iv, const double *temp, const double *ca, const
  * previous value.  Used by tests and the unfused comparator. */
 int swd_set_fd_path(int mode);
 
+/* Time blocking of the fused FD kernels: nt_block steps per launch (1..6;
+ * 1 = one launch per time step) and band rows of the time-blocked kernels.
+ * Results do not depend on either setting (bitwise). */
+int swd_set_fd_time_block(int nt_block, int rows);
+
 /*
  * Model preparation of the wrappers: core/staggerfd_eigen.m:37-107 (with
  * core/calparam.m:3-5, core/pad.m:3-9, core/damp_circle.m:5-67) for
@@ -185,11 +190,19 @@ int swd_engine_forward_picks(swd_engine *e, double *res_r, double *res_l, double
 /* 0 = fused batched kernels (default); 1 = the plain per-statement kernels,
  * one shot per launch (the unfused comparator).  Returns the previous mode. */
 int swd_engine_set_fd_path(swd_engine *e, int mode);
+/* One stage of swd_engine_gradient on the resident buffers: 1 forward FD +
+ * snapshots, 2 Radon + picks, 3 adjoint sources, 4 adjoint FD + imaging,
+ * 5 reduction + dk_vs.  swd_engine_set_picks overwrites the picks between
+ * stages 2 and 3 (npair x nshots each; NULL keeps a side). */
+int swd_engine_run(swd_engine *e, int stage);
+int swd_engine_set_picks(swd_engine *e, const double *cr_r, const double *cr_l);
 int swd_engine_set_tile_rows(swd_engine *e, int tz);
+int swd_engine_set_time_block(swd_engine *e, int nt_block, int rows);
 /* vals[0..9] = nzbc, nxbc, pitch, nt/in_wf, npair, np, device bytes, ccn, lf, tile rows */
 int swd_engine_info(swd_engine *e, long long *vals, int n);
-/* ms[0..5] = forward, dispersion, adjoint, assembly (ms) of the last gradient,
- * mean forward / adjoint march-kernel duration (us, event-timed sample) */
+/* ms[0..6] = forward, dispersion, adjoint, assembly (ms) of the last gradient,
+ * mean forward / adjoint march-kernel launch duration (us, event-timed
+ * sample), time steps advanced per sampled launch */
 int swd_engine_timings(swd_engine *e, double *ms, int n);
 /* Copy a device buffer to host: 0 gathers [shot][t][g] f32, 1 adjoint sources
  * [shot][t][g] f32, 2 {dk_vs, sum cl_img, sum cm_img, sum illum} nz x nx f64,

@@ -77,7 +77,7 @@ def lib() -> ctypes.CDLL:
             fn.argtypes = [P] * 16
             fn = getattr(L, f"oracle_batch_{suf}")
             fn.restype = ctypes.c_int
-            fn.argtypes = [P] * 12 + [ctypes.c_int, P, P, P, P, P, ctypes.c_int]
+            fn.argtypes = [P] * 11 + [ctypes.c_int, P, P, P, P, P, ctypes.c_int]
         L.oracle_prep.restype = ctypes.c_int
         L.oracle_prep.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                   ctypes.c_double, ctypes.c_int, ctypes.c_double] + [P] * 7

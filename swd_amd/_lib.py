@@ -39,6 +39,7 @@ SIGNATURES = {
     "swd_e2drtm_f32": (_I, [_P] * 16),
     "swd_e2drtm_f64": (_I, [_P] * 16),
     "swd_set_fd_path": (_I, [_I]),
+    "swd_set_fd_time_block": (_I, [_I, _I]),
     "swd_prepare_model": (_I, [_P, _P, _I, _I, _I, _D, _D, _I] + [_P] * 7),
     "swd_ricker": (_I, [_D, _D, _I, _P]),
     "swd_radon": (_I, [_I, _P, _I, _I, _I, _D, _D, _I, _D, _D, _D, _D, _D, _I, _I, _I, _P, _P, _P, _P, _P]),
@@ -55,7 +56,10 @@ SIGNATURES = {
     "swd_engine_gradient": (_I, [_P, _P, _P, _P, _P, _P]),
     "swd_engine_forward_picks": (_I, [_P, _P, _P, _P, _P]),
     "swd_engine_set_fd_path": (_I, [_P, _I]),
+    "swd_engine_run": (_I, [_P, _I]),
+    "swd_engine_set_picks": (_I, [_P, _P, _P]),
     "swd_engine_set_tile_rows": (_I, [_P, _I]),
+    "swd_engine_set_time_block": (_I, [_P, _I, _I]),
     "swd_engine_info": (_I, [_P, _P, _I]),
     "swd_engine_timings": (_I, [_P, _P, _I]),
     "swd_engine_read": (_I, [_P, _I, _P, ctypes.c_longlong]),

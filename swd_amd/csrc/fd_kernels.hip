@@ -46,13 +46,13 @@ __device__ __forceinline__ T shfl_from_right(T v) { return __shfl_down(v, 1, kMa
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_march(
-    const FdJob<T>* __restrict__ jobs, const BlockMap* __restrict__ bmap, int step)
+    const FdJob<T>* __restrict__ jobs, const BlockMap* __restrict__ bmap, int step, int launch)
 {
     const BlockMap bm = bmap[blockIdx.x];
     const FdJob<T>& J = jobs[bm.job];
     const int it = step - J.start;
     if (it < 0 || it >= J.nt) return;
-    const int rd = it & 1, wr = rd ^ 1;  // ping-pong buffers of this job's step
+    const int rd = (launch - J.launch0) & 1, wr = rd ^ 1;  // ping-pong buffers of this launch
     const int lane = threadIdx.x & (kMarchLanes - 1);
     const int tile = bm.tile0 + (threadIdx.x >> 6);
     if (tile >= J.tiles_x * J.tiles_z) return;
@@ -194,14 +194,14 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_march(
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_march(
-    const FdJob<T>* __restrict__ jobs, const BlockMap* __restrict__ bmap, int step)
+    const FdJob<T>* __restrict__ jobs, const BlockMap* __restrict__ bmap, int step, int launch)
 {
     const BlockMap bm = bmap[blockIdx.x];
     const FdJob<T>& J = jobs[bm.job];
     const int k = step - J.start;
     const int it = J.nt - 2 - k;
     if (k < 0 || it < 0) return;
-    const int rd = k & 1, wr = rd ^ 1;
+    const int rd = (launch - J.launch0) & 1, wr = rd ^ 1;
     const int lane = threadIdx.x & (kMarchLanes - 1);
     const int tile = bm.tile0 + (threadIdx.x >> 6);
     if (tile >= J.tiles_x * J.tiles_z) return;
@@ -329,24 +329,26 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_march(
 // launchers
 // ---------------------------------------------------------------------------
 template <typename T>
-int launch_fwd_march(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step)
+int launch_fwd_march(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step,
+                     int launch)
 {
-    fd_fwd_march<T><<<nblocks, kMarchLanes * kMarchWaves, 0, st>>>(d_jobs, d_bmap, step);
+    fd_fwd_march<T><<<nblocks, kMarchLanes * kMarchWaves, 0, st>>>(d_jobs, d_bmap, step, launch);
     SWD_HIP(hipGetLastError());
     return kOk;
 }
 
 template <typename T>
-int launch_adj_march(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step)
+int launch_adj_march(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step,
+                     int launch)
 {
-    fd_adj_march<T><<<nblocks, kMarchLanes * kMarchWaves, 0, st>>>(d_jobs, d_bmap, step);
+    fd_adj_march<T><<<nblocks, kMarchLanes * kMarchWaves, 0, st>>>(d_jobs, d_bmap, step, launch);
     SWD_HIP(hipGetLastError());
     return kOk;
 }
 
-template int launch_fwd_march<float>(hipStream_t, const FdJob<float>*, const BlockMap*, int, int);
-template int launch_fwd_march<double>(hipStream_t, const FdJob<double>*, const BlockMap*, int, int);
-template int launch_adj_march<float>(hipStream_t, const FdJob<float>*, const BlockMap*, int, int);
-template int launch_adj_march<double>(hipStream_t, const FdJob<double>*, const BlockMap*, int, int);
+template int launch_fwd_march<float>(hipStream_t, const FdJob<float>*, const BlockMap*, int, int, int);
+template int launch_fwd_march<double>(hipStream_t, const FdJob<double>*, const BlockMap*, int, int, int);
+template int launch_adj_march<float>(hipStream_t, const FdJob<float>*, const BlockMap*, int, int, int);
+template int launch_adj_march<double>(hipStream_t, const FdJob<double>*, const BlockMap*, int, int, int);
 
 }  // namespace swd

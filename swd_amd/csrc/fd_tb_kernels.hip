@@ -1,0 +1,358 @@
+// Temporally blocked SWD FD march kernels for gfx950 (MI355X).
+//
+// One launch advances every shot of a batch by NT time steps.  A wavefront
+// owns 64 consecutive columns and marches down a band of rows; the NT time
+// levels run as a pipeline in registers: level l handles row i - l while
+// level 0 loads row i, so level l always sees rows r-1, r, r+1 of level l-1.
+// Each level shrinks the valid lanes by one on each side (the stencil
+// radius), so a wavefront writes 64 - 2 NT columns, and a band of H output
+// rows loads H + 2 NT + 1 rows.  Intermediate levels never touch memory:
+// per output cell and NT steps the kernel moves (H+2NT+1)/H x 11 loads and
+// 5 stores instead of NT x 16, which is what lifts the stencil above the
+// single-step HBM bound.
+//
+// Per cell and level the arithmetic is exactly that of the single-step
+// march (fd_kernels.hip), in the same order, so results are bitwise equal.
+// Only source type 5 (the hard vertical-velocity source SWD uses) is
+// handled here; other source types go through the single-step kernels.
+#include "swd_internal.h"
+
+namespace swd {
+
+template <typename T>
+__device__ __forceinline__ T tb_left(T v) { return __shfl_up(v, 1, kMarchLanes); }
+template <typename T>
+__device__ __forceinline__ T tb_right(T v) { return __shfl_down(v, 1, kMarchLanes); }
+
+// coefficient record of one row for this lane
+template <typename T>
+struct CoefRow {
+    T temp, b, b1, ca, cl, cm1;
+};
+
+template <typename T, int NT>
+__global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJob<T>* __restrict__ jobs,
+                                                                      const BlockMap* __restrict__ bmap, int step,
+                                                                      int launch)
+{
+    constexpr int W = kMarchLanes - 2 * NT;  // output columns per wavefront
+    const BlockMap bm = bmap[blockIdx.x];
+    const FdJob<T>& J = jobs[bm.job];
+    const int it0 = step - J.start;
+    if (it0 < 0 || it0 >= J.nt) return;
+    const int lane = threadIdx.x & (kMarchLanes - 1);
+    const int tile = bm.tile0 + (threadIdx.x >> 6);
+    if (tile >= J.tiles_x * J.tiles_z) return;
+    const int rd = (launch - J.launch0) & 1, wr = rd ^ 1;
+    const int tix = tile % J.tiles_x, tiz = tile / J.tiles_x;
+    const int nzbc = J.nzbc, nxbc = J.nxbc;
+    const size_t P = (size_t)J.pitch;
+    const int x = tix * W + 1 - NT + lane;
+    const bool xload = x >= 0 && x < nxbc;
+    const bool xin = x >= 1 && x <= nxbc - 2;
+    const bool out = xin && lane >= NT && lane < NT + W;
+    const int xs = xload ? x : 0;
+    const int z0 = 1 + tiz * J.tz;
+    const int z1 = min(z0 + J.tz, nzbc - 1);
+
+    const T* __restrict__ uu0 = J.uu[rd];
+    const T* __restrict__ ww0 = J.ww[rd];
+    const T* __restrict__ xx0 = J.xx[rd];
+    const T* __restrict__ zz0 = J.zz[rd];
+    const T* __restrict__ xz0 = J.xz[rd];
+    const T dtx = J.dtx;
+    const int sz = J.sz, sx = J.sx, pt = J.pt, gz = J.gz;
+    const bool src5 = J.stype == 5;
+    const int nt = J.nt;
+
+    // per-level step, source value and validity
+    T sv[NT];
+    bool valid[NT];
+#pragma unroll
+    for (int l = 0; l < NT; ++l) {
+        const int itl = it0 + l;
+        valid[l] = itl < nt;
+        sv[l] = valid[l] ? J.src[itl] : T(0);
+    }
+    int gi = -1;
+    {
+        const int d = x - J.gx;
+        if (d >= 0 && d % J.dg == 0 && d / J.dg < J.ng) gi = d / J.dg;
+    }
+    const bool xint = x >= J.nbc && x < J.nbc + J.nx;
+
+    // carried per level l (1..NT, stored at index l-1)
+    T cxx[NT], czz[NT], cxz[NT], cwin[NT], cuo[NT], cwo[NT];
+#pragma unroll
+    for (int l = 0; l < NT; ++l) cxx[l] = czz[l] = cxz[l] = cwin[l] = cuo[l] = cwo[l] = T(0);
+    CoefRow<T> ring[NT + 1];
+#pragma unroll
+    for (int k = 0; k <= NT; ++k) ring[k] = CoefRow<T>{T(0), T(0), T(0), T(0), T(0), T(0)};
+
+    const int i0 = z0 - NT - 1, i1 = z1 - 1 + NT;
+    for (int i = i0; i <= i1; ++i) {
+        // level 0: row i of the stored state and its coefficients
+        T ixx = T(0), izz = T(0), ixz = T(0), iu = T(0), iw = T(0);
+        CoefRow<T> cr{T(0), T(0), T(0), T(0), T(0), T(0)};
+        if (i >= 0 && i < nzbc && xload) {
+            const size_t o = (size_t)i * P + xs;
+            ixx = xx0[o]; izz = zz0[o]; ixz = xz0[o]; iu = uu0[o]; iw = ww0[o];
+            cr = CoefRow<T>{J.temp[o], J.b[o], J.b1[o], J.ca[o], J.cl[o], J.cm1[o]};
+        }
+#pragma unroll
+        for (int k = NT; k > 0; --k) ring[k] = ring[k - 1];
+        ring[0] = cr;
+
+#pragma unroll
+        for (int l = 0; l < NT; ++l) {
+            // level l+1 consumes level-l row q = i - l, produces row r = q - 1
+            const int q = i - l, r = q - 1;
+            const CoefRow<T>& cq = ring[l];
+            const CoefRow<T>& cr_ = ring[l + 1];
+            const bool qin = xin && q >= 1 && q <= nzbc - 2;
+            const bool rin = xin && r >= 1 && r <= nzbc - 2;
+            // uu_new(q)
+            const T xx_r = tb_right(ixx);
+            const T u_q = qin ? cq.temp * iu + cq.b * (xx_r - ixx + ixz - cxz[l]) : T(0);
+            // ww_new(r)
+            const T xz_l = tb_left(cxz[l]);
+            T w_r = rin ? cr_.temp * cwin[l] + cr_.b1 * (cxz[l] - xz_l + izz - czz[l]) : T(0);
+            if (src5 && r == sz && x == sx) w_r = sv[l];
+            // stress(r)
+            const T u_r = cuo[l];
+            const T uu_l = tb_left(u_r);
+            const T ww_rt = tb_right(w_r);
+            T oxx = T(0), ozz = T(0), oxz = T(0);
+            const T fux = u_r - uu_l;
+            const T fuz = u_q - u_r;
+            const T bwx = ww_rt - w_r;
+            const T bwz = w_r - cwo[l];
+            if (rin) {
+                const T tc = cr_.temp;
+                oxx = tc * cxx[l] + (cr_.ca * fux + cr_.cl * bwz) * dtx;
+                ozz = (r == pt) ? T(0) : tc * czz[l] + (cr_.ca * bwz + cr_.cl * fux) * dtx;
+                oxz = tc * cxz[l] + (cr_.cm1 * (fuz + bwx)) * dtx;
+            }
+            // records of level l+1 (step it0 + l) owned by this wavefront
+            if (out && valid[l] && r >= z0 && r < z1) {
+                const int itl = it0 + l;
+                if (r == gz && gi >= 0) {
+                    const size_t ti = (size_t)itl * J.ng + gi;
+                    J.trace_w[ti] = w_r;
+                    if (J.trace_u) J.trace_u[ti] = u_r;
+                }
+                if (J.nsf > 0 && (itl % J.in_wf) == 0 && xint && r > pt && r <= pt + J.nz) {
+                    const size_t plane = (size_t)J.nz * J.nx;
+                    const size_t so = (size_t)(itl / J.in_wf) * J.nsf * plane + (size_t)(r - pt - 1) * J.nx +
+                                      (x - J.nbc);
+                    if (J.nsf == 3) {
+                        J.snap[so] = fux;
+                        J.snap[so + plane] = bwz;
+                        J.snap[so + 2 * plane] = fuz + bwx;
+                    } else {
+                        J.snap[so] = fux;
+                        J.snap[so + plane] = fuz;
+                        J.snap[so + 2 * plane] = bwx;
+                        J.snap[so + 3 * plane] = bwz;
+                    }
+                }
+            }
+            // carry this level's inputs and outputs to the next iteration
+            cxx[l] = ixx; czz[l] = izz; cxz[l] = ixz; cwin[l] = iw;
+            cuo[l] = u_q;
+            cwo[l] = w_r;
+            // level-(l+1) row r becomes the input of the next level
+            ixx = oxx; izz = ozz; ixz = oxz; iu = u_r; iw = w_r;
+        }
+        // final level: row i - NT
+        const int rf = i - NT;
+        if (out && rf >= z0 && rf < z1) {
+            const size_t o = (size_t)rf * P + xs;
+            J.xx[wr][o] = ixx;
+            J.zz[wr][o] = izz;
+            J.xz[wr][o] = ixz;
+            J.uu[wr][o] = iu;
+            J.ww[wr][o] = iw;
+        }
+    }
+}
+
+// Adjoint (reverse-time) counterpart with the imaging condition.  Level l
+// runs adjoint step it = nt - 2 - (k0 + l).
+template <typename T, int NT>
+__global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJob<T>* __restrict__ jobs,
+                                                                      const BlockMap* __restrict__ bmap, int step,
+                                                                      int launch)
+{
+    constexpr int W = kMarchLanes - 2 * NT;
+    const BlockMap bm = bmap[blockIdx.x];
+    const FdJob<T>& J = jobs[bm.job];
+    const int k0 = step - J.start;
+    if (k0 < 0 || J.nt - 2 - k0 < 0) return;
+    const int lane = threadIdx.x & (kMarchLanes - 1);
+    const int tile = bm.tile0 + (threadIdx.x >> 6);
+    if (tile >= J.tiles_x * J.tiles_z) return;
+    const int rd = (launch - J.launch0) & 1, wr = rd ^ 1;
+    const int tix = tile % J.tiles_x, tiz = tile / J.tiles_x;
+    const int nzbc = J.nzbc, nxbc = J.nxbc;
+    const size_t P = (size_t)J.pitch;
+    const int x = tix * W + 1 - NT + lane;
+    const bool xload = x >= 0 && x < nxbc;
+    const bool xin = x >= 1 && x <= nxbc - 2;
+    const bool out = xin && lane >= NT && lane < NT + W;
+    const int xs = xload ? x : 0;
+    const int z0 = 1 + tiz * J.tz;
+    const int z1 = min(z0 + J.tz, nzbc - 1);
+
+    const T* __restrict__ uu0 = J.uu[rd];
+    const T* __restrict__ ww0 = J.ww[rd];
+    const T* __restrict__ xx0 = J.xx[rd];
+    const T* __restrict__ zz0 = J.zz[rd];
+    const T* __restrict__ xz0 = J.xz[rd];
+    const T dtx = J.dtx;
+    const int pt = J.pt, gz = J.gz;
+
+    int gi = -1;
+    {
+        const int d = x - J.gx;
+        if (d >= 0 && d % J.dg == 0 && d / J.dg < J.ng) gi = d / J.dg;
+    }
+    int itl[NT];
+    bool valid[NT];
+    T inj[NT];
+#pragma unroll
+    for (int l = 0; l < NT; ++l) {
+        itl[l] = J.nt - 2 - (k0 + l);
+        valid[l] = itl[l] >= 0;
+        inj[l] = (valid[l] && gi >= 0) ? J.dt * J.src[(size_t)itl[l] * J.ng + gi] : T(0);
+    }
+    const bool xint = x >= J.nbc && x < J.nbc + J.nx;
+
+    // carried per level: previous input row (stress, its coefficient products,
+    // old ww), previous outputs u_l(r), w_l(r-1)
+    T cxx[NT], czz[NT], cxz[NT], cwin[NT], cclxx[NT], ccazz[NT], ccmxz[NT], cuo[NT], cwo[NT];
+#pragma unroll
+    for (int l = 0; l < NT; ++l)
+        cxx[l] = czz[l] = cxz[l] = cwin[l] = cclxx[l] = ccazz[l] = ccmxz[l] = cuo[l] = cwo[l] = T(0);
+    CoefRow<T> ring[NT + 1];
+#pragma unroll
+    for (int k = 0; k <= NT; ++k) ring[k] = CoefRow<T>{T(0), T(0), T(0), T(0), T(0), T(0)};
+
+    const int i0 = z0 - NT - 1, i1 = z1 - 1 + NT;
+    for (int i = i0; i <= i1; ++i) {
+        T ixx = T(0), izz = T(0), ixz = T(0), iu = T(0), iw = T(0);
+        CoefRow<T> cr{T(0), T(0), T(0), T(0), T(0), T(0)};
+        if (i >= 0 && i < nzbc && xload) {
+            const size_t o = (size_t)i * P + xs;
+            ixx = xx0[o]; izz = zz0[o]; ixz = xz0[o]; iu = uu0[o]; iw = ww0[o];
+            cr = CoefRow<T>{J.temp[o], J.b[o], J.b1[o], J.ca[o], J.cl[o], J.cm1[o]};
+        }
+#pragma unroll
+        for (int k = NT; k > 0; --k) ring[k] = ring[k - 1];
+        ring[0] = cr;
+
+#pragma unroll
+        for (int l = 0; l < NT; ++l) {
+            const int q = i - l, r = q - 1;
+            const CoefRow<T>& cq = ring[l];
+            const CoefRow<T>& cr_ = ring[l + 1];
+            const bool qin = xin && q >= 1 && q <= nzbc - 2;
+            const bool rin = xin && r >= 1 && r <= nzbc - 2;
+            // coefficient products of input row q (zero where the lane is outside the grid)
+            const T caxx = cq.ca * ixx, clzz = cq.cl * izz, clxx = cq.cl * ixx, cazz = cq.ca * izz,
+                    cmxz = cq.cm1 * ixz;
+            // uu_b(q)
+            const T caxx_l = tb_left(caxx), clzz_l = tb_left(clzz);
+            const T u_q = qin ? cq.temp * iu - cq.b * ((caxx - caxx_l) + (clzz - clzz_l) + (cmxz - ccmxz[l])) : T(0);
+            // ww_b(r) (+ adjoint source at the receiver row)
+            const T cmxz_r = tb_right(ccmxz[l]);
+            T w_r = rin ? cr_.temp * cwin[l] -
+                              cr_.b1 * ((clxx - cclxx[l]) + (cazz - ccazz[l]) + (cmxz_r - ccmxz[l]))
+                        : T(0);
+            if (rin && r == gz) w_r += inj[l];
+            // adjoint stress(r)
+            const T u_r = cuo[l];
+            const T u_rt = tb_right(u_r);
+            const T w_l = tb_left(w_r);
+            T oxx = T(0), ozz = T(0), oxz = T(0);
+            if (rin) {
+                const T tc = cr_.temp;
+                oxx = tc * cxx[l] - dtx * (u_rt - u_r);
+                ozz = (r == pt) ? T(0) : tc * czz[l] - dtx * (w_r - cwo[l]);
+                oxz = tc * cxz[l] - dtx * (u_q - u_r) - dtx * (w_r - w_l);
+            }
+            if (out && valid[l] && r >= z0 && r < z1 && (itl[l] % J.in_wf) == 0 && xint && r > pt &&
+                r <= pt + J.nz) {
+                const size_t plane = (size_t)J.nz * J.nx;
+                const size_t io = (size_t)(r - pt - 1) * J.nx + (x - J.nbc);
+                const T* sn = J.snap + (size_t)(itl[l] / J.in_wf) * J.nsf * plane + io;
+                double Fux, Fwz, Fsh;
+                if (J.nsf == 3) {
+                    Fux = sn[0]; Fwz = sn[plane]; Fsh = sn[2 * plane];
+                } else {
+                    Fux = sn[0]; Fwz = sn[3 * plane]; Fsh = (double)(sn[plane] + sn[2 * plane]);
+                }
+                const double X = oxx, Z = ozz, XZ = oxz;
+                double* im = J.img + io;
+                im[0] += (Fux + Fwz) * (X + Z);
+                im[plane] += 2.0 * (X * Fux + Z * Fwz + 0.5 * XZ * Fsh);
+                im[2 * plane] += (Fux + Fwz) * (Fux + Fwz);
+            }
+            cxx[l] = ixx; czz[l] = izz; cxz[l] = ixz; cwin[l] = iw;
+            cclxx[l] = clxx; ccazz[l] = cazz; ccmxz[l] = cmxz;
+            cuo[l] = u_q;
+            cwo[l] = w_r;
+            ixx = oxx; izz = ozz; ixz = oxz; iu = u_r; iw = w_r;
+        }
+        const int rf = i - NT;
+        if (out && rf >= z0 && rf < z1) {
+            const size_t o = (size_t)rf * P + xs;
+            J.xx[wr][o] = ixx;
+            J.zz[wr][o] = izz;
+            J.xz[wr][o] = ixz;
+            J.uu[wr][o] = iu;
+            J.ww[wr][o] = iw;
+        }
+    }
+}
+
+template <typename T>
+int launch_fwd_tb(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step, int launch,
+                  int nt_block)
+{
+    const dim3 blk(kMarchLanes * kMarchWaves);
+    switch (nt_block) {
+    case 2: fd_fwd_tb<T, 2><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    case 3: fd_fwd_tb<T, 3><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    case 4: fd_fwd_tb<T, 4><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    case 5: fd_fwd_tb<T, 5><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    case 6: fd_fwd_tb<T, 6><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    default: set_error("unsupported time block %d", nt_block); return kErrArg;
+    }
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
+template <typename T>
+int launch_adj_tb(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step, int launch,
+                  int nt_block)
+{
+    const dim3 blk(kMarchLanes * kMarchWaves);
+    switch (nt_block) {
+    case 2: fd_adj_tb<T, 2><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    case 3: fd_adj_tb<T, 3><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    case 4: fd_adj_tb<T, 4><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    case 5: fd_adj_tb<T, 5><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    case 6: fd_adj_tb<T, 6><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    default: set_error("unsupported time block %d", nt_block); return kErrArg;
+    }
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
+template int launch_fwd_tb<float>(hipStream_t, const FdJob<float>*, const BlockMap*, int, int, int, int);
+template int launch_adj_tb<float>(hipStream_t, const FdJob<float>*, const BlockMap*, int, int, int, int);
+template int launch_fwd_tb<double>(hipStream_t, const FdJob<double>*, const BlockMap*, int, int, int, int);
+template int launch_adj_tb<double>(hipStream_t, const FdJob<double>*, const BlockMap*, int, int, int, int);
+
+}  // namespace swd

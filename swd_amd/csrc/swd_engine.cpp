@@ -209,41 +209,12 @@ int run_fd_batch(const std::vector<Engine*>& engines, bool adj, bool snapshots, 
         if (adj) SWD_HIP(hipMemsetAsync(e->imgs.p, 0, e->imgs.bytes, st));
         else SWD_HIP(hipMemsetAsync(e->traces.p, 0, e->traces.bytes, st));
     }
-    MarchPlan plan;
-    int rc = build_march_plan(jobs, tz, plan);
-    if (rc) return rc;
-    DevBuf dj, db;
-    if ((rc = dj.upload(jobs.data(), jobs.size() * sizeof(FdJob<float>), st)) ||
-        (rc = db.upload(plan.bmap.data(), plan.bmap.size() * sizeof(BlockMap), st)))
-        return rc;
-    const FdJob<float>* J = dj.as<FdJob<float>>();
-    const BlockMap* B = db.as<BlockMap>();
-    // every 37th launch (a stride coprime to the snapshot interval) is bracketed
-    // by events on this stream: their mean is the per-launch kernel duration
-    const int stride = 37;
-    const int nsamp = (steps + stride - 1) / stride;
-    std::vector<hipEvent_t> ev(2 * nsamp);
-    for (auto& e : ev) SWD_HIP(hipEventCreate(&e));
-    for (int step = 0; step < steps; ++step) {
-        const bool samp = step % stride == 0;
-        const int si = step / stride;
-        if (samp) SWD_HIP(hipEventRecord(ev[2 * si], st));
-        rc = adj ? launch_adj_march<float>(st, J, B, plan.nblocks, step)
-                 : launch_fwd_march<float>(st, J, B, plan.nblocks, step);
-        if (rc) break;
-        if (samp) SWD_HIP(hipEventRecord(ev[2 * si + 1], st));
-    }
-    const hipError_t se = hipStreamSynchronize(st);
-    float tot = 0;
-    for (int i = 0; i < nsamp && rc == 0; ++i) {
-        float ms = 0;
-        if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) == hipSuccess) tot += ms;
-    }
-    for (auto& e : ev) (void)hipEventDestroy(e);
-    if (rc) return rc;
-    if (se != hipSuccess) { set_error("FD batch failed: %s", hipGetErrorString(se)); return kErrHip; }
-    if (kernel_us) *kernel_us = nsamp > 0 ? 1000.f * tot / nsamp : 0.f;
-    return kOk;
+    const Engine* e0 = engines[0];
+    int nt_block = 1;
+    const int rc = run_march<float>(st, jobs, adj, steps, e0->nt_block, e0->tb_rows, tz, kernel_us, &nt_block);
+    if (rc == kOk)
+        for (Engine* e : engines) e->times.steps_per_launch = nt_block;
+    return rc;
 }
 
 int Engine::run_fd(bool adj, bool snapshots)
@@ -440,6 +411,50 @@ int swd_engine_set_tile_rows(swd_engine* e, int tz)
     return kOk;
 }
 
+int swd_engine_run(swd_engine* e, int stage)
+{
+    if (!e) return kErrArg;
+    Engine& g = e->e;
+    int rc;
+    switch (stage) {
+    case 1: return g.forward(true);
+    case 2:
+        if ((rc = g.dispersion())) return rc;
+        SWD_HIP(hipStreamSynchronize(g.st));
+        return kOk;
+    case 3:
+        if ((rc = g.adjoint_source())) return rc;
+        SWD_HIP(hipStreamSynchronize(g.st));
+        return kOk;
+    case 4: return g.adjoint();
+    case 5:
+        if ((rc = g.assemble())) return rc;
+        SWD_HIP(hipStreamSynchronize(g.st));
+        return kOk;
+    default: set_error("unknown stage %d", stage); return kErrArg;
+    }
+}
+
+int swd_engine_set_picks(swd_engine* e, const double* cr_r, const double* cr_l)
+{
+    if (!e) return kErrArg;
+    Engine& g = e->e;
+    const size_t n = (size_t)g.npair * g.p.nshots;
+    if (cr_r) SWD_HIP(hipMemcpyAsync(g.disp.crpre.p, cr_r, n * sizeof(double), hipMemcpyHostToDevice, g.st));
+    if (cr_l)
+        SWD_HIP(hipMemcpyAsync((double*)g.disp.crpre.p + n, cr_l, n * sizeof(double), hipMemcpyHostToDevice, g.st));
+    SWD_HIP(hipStreamSynchronize(g.st));
+    return kOk;
+}
+
+int swd_engine_set_time_block(swd_engine* e, int nt_block, int rows)
+{
+    if (!e || nt_block < 1 || nt_block > 6 || rows < 1) { set_error("time block must be 1..6 steps, rows >= 1"); return kErrArg; }
+    e->e.nt_block = nt_block;
+    e->e.tb_rows = rows;
+    return kOk;
+}
+
 int swd_engine_info(swd_engine* e, long long* vals, int n)
 {
     if (!e || !vals) return kErrArg;
@@ -454,8 +469,9 @@ int swd_engine_timings(swd_engine* e, double* ms, int n)
 {
     if (!e || !ms) return kErrArg;
     const StageTimes& t = e->e.times;
-    const double v[6] = {t.fwd_ms, t.disp_ms, t.adj_ms, t.asm_ms, t.fwd_kernel_us, t.adj_kernel_us};
-    for (int i = 0; i < n && i < 6; ++i) ms[i] = v[i];
+    const double v[7] = {t.fwd_ms, t.disp_ms, t.adj_ms, t.asm_ms, t.fwd_kernel_us, t.adj_kernel_us,
+                         (double)t.steps_per_launch};
+    for (int i = 0; i < n && i < 7; ++i) ms[i] = v[i];
     return kOk;
 }
 

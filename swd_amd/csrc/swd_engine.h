@@ -18,6 +18,7 @@ int launch_assemble_gradient(hipStream_t st, const double* imgs, int nshots, int
 struct StageTimes {
     float fwd_ms = 0, disp_ms = 0, adj_ms = 0, asm_ms = 0;
     float fwd_kernel_us = 0, adj_kernel_us = 0;  // mean duration of one march launch (sampled)
+    int steps_per_launch = 1;                    // time steps advanced by one sampled launch
     int fwd_samples = 0, adj_samples = 0;
 };
 
@@ -30,7 +31,9 @@ public:
     hipStream_t st = nullptr;
     bool own_stream = false;
     int fd_path = 0;  // 0 fused batched, 1 plain one-shot-per-launch
-    int tz = 16;
+    int tz = 16;        // band rows of the single-step march
+    int nt_block = 4;   // time steps per time-blocked launch (1: single-step kernels only)
+    int tb_rows = 32;   // band rows of the time-blocked march
     bool have_model = false, have_wavelet = false;
     double vpmax = 0;
 

@@ -137,7 +137,7 @@ static int staggerfd_impl(const T* iv, const T* const coef_in[7], const T* s, T*
 
     Stream st;
     if ((rc = st.create())) return rc;
-    DevBuf coefs, state, strain, traces, snaps, src, djob, dbmap;
+    DevBuf coefs, state, strain, traces, snaps, src;
     // coefficient order in the device block: temp, b, b1, ca, cl, cm1
     const int map[6] = {0, 5, 6, 1, 2, 4};
     {
@@ -177,14 +177,9 @@ static int staggerfd_impl(const T* iv, const T* const coef_in[7], const T* s, T*
     J.dtx = (T)d.dtx; J.dt = T(0);
     if (fused) {
         std::vector<FdJob<T>> jobs{J};
-        MarchPlan plan;
-        if ((rc = build_march_plan(jobs, fd_tile_rows(), plan))) return rc;
-        if ((rc = djob.upload(jobs.data(), sizeof(FdJob<T>), st.s)) ||
-            (rc = dbmap.upload(plan.bmap.data(), plan.bmap.size() * sizeof(BlockMap), st.s)))
+        if ((rc = run_march<T>(st.s, jobs, false, d.nt, fd_time_block(), fd_tb_rows(), fd_tile_rows(), nullptr,
+                               nullptr)))
             return rc;
-        for (int step = 0; step < d.nt; ++step)
-            if ((rc = launch_fwd_march<T>(st.s, (const FdJob<T>*)djob.p, (const BlockMap*)dbmap.p, plan.nblocks, step)))
-                return rc;
     } else {
         if ((rc = strain.alloc(4 * plane * sizeof(T)))) return rc;
         SWD_HIP(hipMemsetAsync(strain.p, 0, 4 * plane * sizeof(T), st.s));
@@ -268,7 +263,7 @@ static int e2drtm_impl(const T* iv, const T* const coef_in[7], const T* adj, con
 
     Stream st;
     if ((rc = st.create())) return rc;
-    DevBuf coefs, state, dadj, snaps, img, djob, dbmap;
+    DevBuf coefs, state, dadj, snaps, img;
     const int map[6] = {0, 5, 6, 1, 2, 4};
     {
         std::vector<T> h(6 * plane);
@@ -318,14 +313,9 @@ static int e2drtm_impl(const T* iv, const T* const coef_in[7], const T* adj, con
     J.dtx = (T)d.dtx; J.dt = (T)d.dt;
     if (fused) {
         std::vector<FdJob<T>> jobs{J};
-        MarchPlan plan;
-        if ((rc = build_march_plan(jobs, fd_tile_rows(), plan))) return rc;
-        if ((rc = djob.upload(jobs.data(), sizeof(FdJob<T>), st.s)) ||
-            (rc = dbmap.upload(plan.bmap.data(), plan.bmap.size() * sizeof(BlockMap), st.s)))
+        if ((rc = run_march<T>(st.s, jobs, true, d.nt - 1, fd_time_block(), fd_tb_rows(), fd_tile_rows(), nullptr,
+                               nullptr)))
             return rc;
-        for (int step = 0; step < d.nt - 1; ++step)
-            if ((rc = launch_adj_march<T>(st.s, (const FdJob<T>*)djob.p, (const BlockMap*)dbmap.p, plan.nblocks, step)))
-                return rc;
     } else {
         for (int it = d.nt - 2; it >= 0; --it)
             if ((rc = plain_adjoint_step<T>(st.s, J, it))) return rc;
@@ -339,28 +329,6 @@ static int e2drtm_impl(const T* iv, const T* const coef_in[7], const T* adj, con
                 img_out[f][(size_t)ix * d.nz + iz] = (T)h[f * splane + (size_t)iz * d.nx + ix];
     return kOk;
 }
-
-// ---------------------------------------------------------------------------
-// march tiling
-// ---------------------------------------------------------------------------
-template <typename T>
-int build_march_plan(std::vector<FdJob<T>>& jobs, int tz, MarchPlan& plan)
-{
-    plan.bmap.clear();
-    for (size_t j = 0; j < jobs.size(); ++j) {
-        FdJob<T>& J = jobs[j];
-        J.tz = tz;
-        J.tiles_x = div_up(J.nxbc - 2, kMarchOut);
-        J.tiles_z = div_up(J.nzbc - 2, tz);
-        const int tiles = J.tiles_x * J.tiles_z;
-        for (int t = 0; t < tiles; t += kMarchWaves) plan.bmap.push_back(BlockMap{(int)j, t});
-    }
-    plan.nblocks = (int)plan.bmap.size();
-    if (plan.nblocks == 0) { set_error("empty FD batch"); return kErrArg; }
-    return kOk;
-}
-template int build_march_plan<float>(std::vector<FdJob<float>>&, int, MarchPlan&);
-template int build_march_plan<double>(std::vector<FdJob<double>>&, int, MarchPlan&);
 
 // ---------------------------------------------------------------------------
 // model preparation (staggerfd_eigen.m:37-107) and Ricker (ricker.m)
@@ -480,6 +448,13 @@ int swd_device_info(int* n_devices, char* arch, int arch_len)
 }
 
 int swd_set_fd_path(int mode) { return g_fd_path.exchange(mode ? 1 : 0); }
+
+int swd_set_fd_time_block(int nt_block, int rows)
+{
+    if (nt_block < 1 || nt_block > 6 || rows < 1) { set_error("time block must be 1..6 steps, rows >= 1"); return kErrArg; }
+    set_fd_time_block(nt_block, rows);
+    return kOk;
+}
 
 int swd_staggerfd_f32(const float* iv, const float* temp, const float* ca, const float* cl, const float* cm,
                       const float* cm1, const float* b, const float* b1, const float* s, float* seismo_u,

@@ -91,13 +91,37 @@ struct MarchPlan {
     int nblocks = 0;
 };
 
+// Tile every job of a batch for a march launch: out_cols output columns per
+// wavefront (62 single-step, 64 - 2 NT time-blocked) and tz rows per band.
 template <typename T>
-int build_march_plan(std::vector<FdJob<T>>& jobs, int tz, MarchPlan& plan);
+int build_march_plan(std::vector<FdJob<T>>& jobs, int tz, MarchPlan& plan, int out_cols = kMarchOut);
 
 template <typename T>
-int launch_fwd_march(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step);
+int launch_fwd_march(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step,
+                     int launch);
 template <typename T>
-int launch_adj_march(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step);
+int launch_adj_march(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step,
+                     int launch);
+template <typename T>
+int launch_fwd_tb(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step, int launch,
+                  int nt_block);
+template <typename T>
+int launch_adj_tb(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step, int launch,
+                  int nt_block);
+
+// Time-blocking settings: steps per launch (1 = single-step kernels) and band
+// rows of the time-blocked kernels; process-wide defaults for the MEX path.
+int fd_time_block();
+int fd_tb_rows();
+void set_fd_time_block(int nt_block, int rows);
+
+// Advance a batch of jobs (all starting together) over `steps` forward or
+// adjoint steps with time-blocked launches plus single-step remainders.
+// kernel_us receives the mean event-timed duration of one launch of the
+// dominant kernel, launches_tb/launches_ss the launch counts.
+template <typename T>
+int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adjoint, int steps, int nt_block, int tb_rows,
+              int ss_rows, float* kernel_us, int* launches_tb);
 template <typename T>
 int plain_forward_step(hipStream_t st, const FdJob<T>& J, T* const w[4], int it);
 template <typename T>

@@ -68,6 +68,7 @@ struct FdJob {
     int gz, gx, dg, ng;     // 0-based receiver row / first column, spacing, count
     int nt, in_wf;          // steps and snapshot interval
     int start;              // launch step at which this job's step 0 runs
+    int launch0;            // launch index of the job's first launch (ping-pong parity)
     int nsf;                // snapshot fields (3 or 4); 0 = no snapshots
     int tiles_x, tiles_z, tz;  // march tiling (fused kernels)
     T dtx, dt;

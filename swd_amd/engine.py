@@ -104,16 +104,33 @@ class Engine:
         check(lib().swd_engine_forward_picks(self.h, ptr(rr), ptr(rl), ptr(cr), ptr(cl)))
         return {"res_r": rr, "res_l": rl, "cr_r": cr, "cr_l": cl}
 
+    STAGES = {"forward": 1, "dispersion": 2, "adjoint_source": 3, "adjoint": 4, "assemble": 5}
+
+    def run(self, stage: str):
+        """One stage of gradient() on the resident buffers."""
+        check(lib().swd_engine_run(self.h, self.STAGES[stage]))
+
+    def set_picks(self, cr_r=None, cr_l=None):
+        """Overwrite the current picks (npair x nshots) between 'dispersion' and 'adjoint_source'."""
+        a = None if cr_r is None else fortran(cr_r, np.float64)
+        b = None if cr_l is None else fortran(cr_l, np.float64)
+        check(lib().swd_engine_set_picks(self.h, ptr(a), ptr(b)))
+
     def set_fd_path(self, mode: int) -> int:
         return lib().swd_engine_set_fd_path(self.h, int(mode))
 
     def set_tile_rows(self, tz: int):
         check(lib().swd_engine_set_tile_rows(self.h, int(tz)))
 
+    def set_time_block(self, nt_block: int, rows: int = 32):
+        """Steps per fused FD launch (1 = single-step kernels) and band rows."""
+        check(lib().swd_engine_set_time_block(self.h, int(nt_block), int(rows)))
+
     def timings(self) -> dict:
-        v = np.zeros(6)
-        check(lib().swd_engine_timings(self.h, ptr(v), 6))
-        return dict(zip(("fwd_ms", "disp_ms", "adj_ms", "asm_ms", "fwd_kernel_us", "adj_kernel_us"), v))
+        v = np.zeros(7)
+        check(lib().swd_engine_timings(self.h, ptr(v), 7))
+        return dict(zip(("fwd_ms", "disp_ms", "adj_ms", "asm_ms", "fwd_kernel_us", "adj_kernel_us",
+                         "steps_per_launch"), v))
 
     def read(self, what: str) -> np.ndarray:
         q = self.problem

@@ -69,6 +69,11 @@ def eigen_e2drtm_double(iv, temp, ca, cl, cm, cm1, b, b1, adj, fux, fuz, bwx, bw
     return _e2drtm(np.float64, load().swd_e2drtm_f64, iv, temp, ca, cl, cm, cm1, b, b1, adj, fux, fuz, bwx, bwz)
 
 
+def set_time_block(nt_block: int, rows: int = 32) -> None:
+    """Steps per fused FD launch of the MEX entry points (1 = single-step kernels)."""
+    check(load().swd_set_fd_time_block(int(nt_block), int(rows)))
+
+
 def set_fd_path(mode: int) -> int:
     """0 = automatic (fused when possible), 1 = plain kernels only. Returns the old mode."""
     return load().swd_set_fd_path(int(mode))

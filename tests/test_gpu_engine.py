@@ -96,6 +96,25 @@ def test_engine_single_shot_gradient_vs_fp64_oracle(single):
     e.close()
 
 
+def test_engine_stages_given_oracle_picks(single):
+    """Adjoint source, adjoint FD and dk_vs on the GPU from the oracle's picks:
+    the FP32 chain downstream of the (integer) picks against FP64."""
+    prob, mod, cr0_r, cr0_l, dk_ref, shots = single
+    o = shots[0]
+    e = _engine(prob, mod, cr0_r, cr0_l)
+    e.run("forward")
+    e.run("dispersion")
+    e.set_picks(o["cr_r"][:, None], o["cr_l"][:, None])
+    e.run("adjoint_source")
+    e.run("adjoint")
+    e.run("assemble")
+    err = {"adjoint_source": rel(e.read("adjoint_sources")[0], o["adj"]),
+           "dk_vs": rel(e.read("gradient")[0], dk_ref)}
+    print("engine stages given oracle picks, fp32 rel-L2 vs fp64:", err)
+    assert max(err.values()) < 1e-4, err
+    e.close()
+
+
 def test_engine_three_shots_vs_oracle():
     nz, nx = 30, 120
     vs = np.tile(np.linspace(220, 420, nz)[:, None], (1, nx))
