@@ -53,7 +53,7 @@ def cpu_baseline(prob, mod, seconds_hint=15.0):
     threads = max(1, min(16, os.cpu_count() or 1))
     nz, nx, nbc = prob["nz"], prob["nx"], prob["nbc"]
     nzbc, nxbc = nz + 2 + nbc, nx + 2 * nbc
-    nt = 1000
+    nt = prob["nt"]
     c = O.prep(mod["vp"], mod["vs"], nbc, prob["dx"], prob["dt"], 22)
     f32 = {k: np.asfortranarray(v, dtype=np.float32) for k, v in c.items()}
     dtx = prob["dt"] / prob["dx"]
