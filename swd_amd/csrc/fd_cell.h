@@ -1,0 +1,56 @@
+// Cell updates of the SWD staggered-grid scheme, written once with explicit
+// fused multiply-adds and shared by every fused FD kernel (single-step and
+// time-blocked), so all of them round identically.  The library is compiled
+// with -ffp-contract=off: no other contraction happens.
+//
+// forward  (core/eigen_staggerfd_single.cpp:224-228, :334-336)
+//   v'   = temp*v + b*d
+//   xx'  = temp*xx + (ca*fux + cl*bwz)*dtx
+//   zz'  = temp*zz + (ca*bwz + cl*fux)*dtx
+//   xz'  = temp*xz + (cm1*(fuz + bwx))*dtx
+// adjoint  (core/eigen_e2drtm_single.cpp:182-187, :245-248)
+//   v'   = temp*v - b*d
+//   s'   = temp*s - dtx*d1 (- dtx*d2)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace swd {
+
+template <typename T>
+__device__ __forceinline__ T cell_vel(T temp, T v, T b, T d)
+{
+    return fma(b, d, temp * v);
+}
+
+template <typename T>
+__device__ __forceinline__ T cell_stress2(T temp, T s, T c1, T e1, T c2, T e2, T dtx)
+{
+    return fma(fma(c1, e1, c2 * e2), dtx, temp * s);
+}
+
+template <typename T>
+__device__ __forceinline__ T cell_shear(T temp, T s, T cm1, T e, T dtx)
+{
+    return fma(cm1 * e, dtx, temp * s);
+}
+
+template <typename T>
+__device__ __forceinline__ T cell_adj_vel(T temp, T v, T b, T d)
+{
+    return fma(-b, d, temp * v);
+}
+
+template <typename T>
+__device__ __forceinline__ T cell_adj_stress(T temp, T s, T dtx, T d)
+{
+    return fma(-dtx, d, temp * s);
+}
+
+template <typename T>
+__device__ __forceinline__ T cell_adj_shear(T temp, T s, T dtx, T d1, T d2)
+{
+    return fma(-dtx, d2, fma(-dtx, d1, temp * s));
+}
+
+}  // namespace swd

@@ -26,6 +26,7 @@
 //   xx = temp*xx + (ca*fux + cl*bwz)*dtx; zz = temp*zz + (ca*bwz + cl*fux)*dtx;
 //   xz = temp*xz + cm1*(fuz+bwx)*dtx;  zz[pt,:] = 0                    (:334-340)
 // Adjoint step, core/eigen_e2drtm_single.cpp:181-283 (it = nt-2 .. 0).
+#include "fd_cell.h"
 #include "swd_internal.h"
 
 namespace swd {
@@ -109,15 +110,15 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_march(
     {
         const T xz_ml = shfl_from_left(xz_m);
         if (r - 1 >= 1 && xin)
-            ww_m = fix_w(temp[(size_t)(r - 1) * P + xs] * ww0[(size_t)(r - 1) * P + xs] +
-                             bb1[(size_t)(r - 1) * P + xs] * (xz_m - xz_ml + zz_c - zz_m),
+            ww_m = fix_w(cell_vel(temp[(size_t)(r - 1) * P + xs], ww0[(size_t)(r - 1) * P + xs],
+                                  bb1[(size_t)(r - 1) * P + xs], xz_m - xz_ml + zz_c - zz_m),
                          r - 1);
     }
     T uu_c;
     {
         const T xx_cr = shfl_from_right(xx_c);
-        uu_c = xin ? fix_u(temp[(size_t)r * P + xs] * uu0[(size_t)r * P + xs] +
-                               bb[(size_t)r * P + xs] * (xx_cr - xx_c + xz_c - xz_m),
+        uu_c = xin ? fix_u(cell_vel(temp[(size_t)r * P + xs], uu0[(size_t)r * P + xs], bb[(size_t)r * P + xs],
+                                    xx_cr - xx_c + xz_c - xz_m),
                            r)
                    : T(0);
     }
@@ -133,11 +134,11 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_march(
         const T xx_nr = shfl_from_right(xx_n);
         T uu_n = T(0);
         if (r + 1 <= nzbc - 2 && xin)
-            uu_n = fix_u(temp[on] * uu0[on] + bb[on] * (xx_nr - xx_n + xz_n - xz_c), r + 1);
+            uu_n = fix_u(cell_vel(temp[on], uu0[on], bb[on], xx_nr - xx_n + xz_n - xz_c), r + 1);
         // ww_new(r)
         const T xz_cl = shfl_from_left(xz_c);
         const T tc = ldin(temp, r);
-        T ww_c = xin ? fix_w(tc * ww0[o] + bb1[o] * (xz_c - xz_cl + zz_n - zz_c), r) : T(0);
+        T ww_c = xin ? fix_w(cell_vel(tc, ww0[o], bb1[o], xz_c - xz_cl + zz_n - zz_c), r) : T(0);
         // strain and stress at row r
         const T uu_l = shfl_from_left(uu_c);
         const T ww_r = shfl_from_right(ww_c);
@@ -152,9 +153,9 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_march(
                 if (stype == 4) zo += sv;
             }
             const T ca = J.ca[o], cl = J.cl[o], cm1 = J.cm1[o];
-            xx1[o] = tc * xo + (ca * fux + cl * bwz) * dtx;
-            zz1[o] = (r == J.pt) ? T(0) : tc * zo + (ca * bwz + cl * fux) * dtx;
-            xz1[o] = tc * xz_c + (cm1 * (fuz + bwx)) * dtx;
+            xx1[o] = cell_stress2(tc, xo, ca, fux, cl, bwz, dtx);
+            zz1[o] = (r == J.pt) ? T(0) : cell_stress2(tc, zo, ca, bwz, cl, fux, dtx);
+            xz1[o] = cell_shear(tc, xz_c, cm1, fuz + bwx, dtx);
             uu1[o] = uu_c;
             ww1[o] = ww_c;
             if (r == J.gz) {
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_march(
         const T cmxz_r = shfl_from_right(pc.cmxz);
         if (!xin || rr < 1 || rr > nzbc - 2) return T(0);
         const size_t o = (size_t)rr * P + xs;
-        T w = temp[o] * ww0[o] - J.b1[o] * ((pn.clxx - pc.clxx) + (pn.cazz - pc.cazz) + (cmxz_r - pc.cmxz));
+        T w = cell_adj_vel(temp[o], ww0[o], J.b1[o], (pn.clxx - pc.clxx) + (pn.cazz - pc.cazz) + (cmxz_r - pc.cmxz));
         if (rr == J.gz) w += inj;
         return w;
     };
@@ -262,7 +263,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_march(
         const T clzz_l = shfl_from_left(pc.clzz);
         if (!xin || rr < 1 || rr > nzbc - 2) return T(0);
         const size_t o = (size_t)rr * P + xs;
-        return temp[o] * uu0[o] - J.b[o] * ((pc.caxx - caxx_l) + (pc.clzz - clzz_l) + (pc.cmxz - pm.cmxz));
+        return cell_adj_vel(temp[o], uu0[o], J.b[o], (pc.caxx - caxx_l) + (pc.clzz - clzz_l) + (pc.cmxz - pm.cmxz));
     };
 
     int r = z0;
@@ -291,9 +292,9 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_march(
         const T ww_l = shfl_from_left(ww_c);
         if (out) {
             const T tc = temp[o];
-            const T xxn = tc * xx_c - dtx * (uu_r - uu_c);
-            const T zzn = (r == J.pt) ? T(0) : tc * zz_c - dtx * (ww_c - ww_m);
-            const T xzn = tc * xz_c - dtx * (uu_n - uu_c) - dtx * (ww_c - ww_l);
+            const T xxn = cell_adj_stress(tc, xx_c, dtx, uu_r - uu_c);
+            const T zzn = (r == J.pt) ? T(0) : cell_adj_stress(tc, zz_c, dtx, ww_c - ww_m);
+            const T xzn = cell_adj_shear(tc, xz_c, dtx, uu_n - uu_c, ww_c - ww_l);
             xx1[o] = xxn;
             zz1[o] = zzn;
             xz1[o] = xzn;

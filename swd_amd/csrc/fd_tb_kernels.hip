@@ -13,16 +13,52 @@
 //
 // Per cell and level the arithmetic is exactly that of the single-step
 // march (fd_kernels.hip), in the same order, so results are bitwise equal.
+// The kernels rely on the coefficient planes being zero on the outer ring
+// (the fused paths pack them so): every ring cell then computes exactly 0,
+// as the reference's never-updated ring, without per-lane predicates.
 // Only source type 5 (the hard vertical-velocity source SWD uses) is
 // handled here; other source types go through the single-step kernels.
+#include "fd_cell.h"
 #include "swd_internal.h"
 
 namespace swd {
 
+// Global-address-space pointers and 32-bit byte offsets: loads and stores of
+// grid rows become global_load/store with a scalar base and a vector offset.
 template <typename T>
-__device__ __forceinline__ T tb_left(T v) { return __shfl_up(v, 1, kMarchLanes); }
+using gptr = __attribute__((address_space(1))) T*;
 template <typename T>
-__device__ __forceinline__ T tb_right(T v) { return __shfl_down(v, 1, kMarchLanes); }
+__device__ __forceinline__ T gld(gptr<const T> base, unsigned byte_off)
+{
+    return *(gptr<const T>)((gptr<const char>)base + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ void gst(gptr<T> base, unsigned byte_off, T v)
+{
+    *(gptr<T>)((gptr<char>)base + byte_off) = v;
+}
+
+// Neighbour-lane exchange with DPP full-wave shifts (a VALU operand
+// modifier, no LDS round trip): wave_shr:1 (0x138) gives lane i the value of
+// lane i-1, wave_shl:1 (0x130) that of lane i+1.  Lanes 0 / 63 receive 0;
+// they are halo lanes whose results are never written.
+__device__ __forceinline__ int dpp_from_left(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_from_right(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+
+__device__ __forceinline__ float tb_left(float v) { return __int_as_float(dpp_from_left(__float_as_int(v))); }
+__device__ __forceinline__ float tb_right(float v) { return __int_as_float(dpp_from_right(__float_as_int(v))); }
+__device__ __forceinline__ double tb_left(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_from_left((int)(b & 0xffffffffLL)), hi = dpp_from_left((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double tb_right(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_from_right((int)(b & 0xffffffffLL)), hi = dpp_from_right((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 
 // coefficient record of one row for this lane
 template <typename T>
@@ -43,7 +79,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
     const int lane = threadIdx.x & (kMarchLanes - 1);
     const int tile = bm.tile0 + (threadIdx.x >> 6);
     if (tile >= J.tiles_x * J.tiles_z) return;
-    const int rd = (launch - J.launch0) & 1, wr = rd ^ 1;
+    const int rd = (launch - J.launch0) & 1;
     const int tix = tile % J.tiles_x, tiz = tile / J.tiles_x;
     const int nzbc = J.nzbc, nxbc = J.nxbc;
     const size_t P = (size_t)J.pitch;
@@ -55,15 +91,29 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
     const int z0 = 1 + tiz * J.tz;
     const int z1 = min(z0 + J.tz, nzbc - 1);
 
-    const T* __restrict__ uu0 = J.uu[rd];
-    const T* __restrict__ ww0 = J.ww[rd];
-    const T* __restrict__ xx0 = J.xx[rd];
-    const T* __restrict__ zz0 = J.zz[rd];
-    const T* __restrict__ xz0 = J.xz[rd];
+    // uniform selections keep the pointers in scalar registers
+    const gptr<const T> uu0 = (gptr<const T>)(rd ? J.uu[1] : J.uu[0]);
+    const gptr<const T> ww0 = (gptr<const T>)(rd ? J.ww[1] : J.ww[0]);
+    const gptr<const T> xx0 = (gptr<const T>)(rd ? J.xx[1] : J.xx[0]);
+    const gptr<const T> zz0 = (gptr<const T>)(rd ? J.zz[1] : J.zz[0]);
+    const gptr<const T> xz0 = (gptr<const T>)(rd ? J.xz[1] : J.xz[0]);
+    const gptr<T> uu1 = (gptr<T>)(rd ? J.uu[0] : J.uu[1]);
+    const gptr<T> ww1 = (gptr<T>)(rd ? J.ww[0] : J.ww[1]);
+    const gptr<T> xx1 = (gptr<T>)(rd ? J.xx[0] : J.xx[1]);
+    const gptr<T> zz1 = (gptr<T>)(rd ? J.zz[0] : J.zz[1]);
+    const gptr<T> xz1 = (gptr<T>)(rd ? J.xz[0] : J.xz[1]);
+    const gptr<const T> ctemp = (gptr<const T>)J.temp, cb = (gptr<const T>)J.b, cb1 = (gptr<const T>)J.b1;
+    const gptr<const T> cca = (gptr<const T>)J.ca, ccl = (gptr<const T>)J.cl, ccm1 = (gptr<const T>)J.cm1;
+    const unsigned rowb = (unsigned)(P * sizeof(T));
+    const unsigned xb = (unsigned)(xs * sizeof(T));
     const T dtx = J.dtx;
     const int sz = J.sz, sx = J.sx, pt = J.pt, gz = J.gz;
     const bool src5 = J.stype == 5;
     const int nt = J.nt;
+    const int ng = J.ng, in_wf = J.in_wf, nsf = J.nsf, nzi = J.nz, nxi = J.nx, nbc = J.nbc;
+    T* const trace_w = J.trace_w;
+    T* const trace_u = J.trace_u;
+    T* const snap = J.snap;
 
     // per-level step, source value and validity
     T sv[NT];
@@ -91,13 +141,13 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
 
     const int i0 = z0 - NT - 1, i1 = z1 - 1 + NT;
     for (int i = i0; i <= i1; ++i) {
-        // level 0: row i of the stored state and its coefficients
+        // level 0: row i of the stored state and its coefficients (0 off the grid)
         T ixx = T(0), izz = T(0), ixz = T(0), iu = T(0), iw = T(0);
         CoefRow<T> cr{T(0), T(0), T(0), T(0), T(0), T(0)};
         if (i >= 0 && i < nzbc && xload) {
-            const size_t o = (size_t)i * P + xs;
-            ixx = xx0[o]; izz = zz0[o]; ixz = xz0[o]; iu = uu0[o]; iw = ww0[o];
-            cr = CoefRow<T>{J.temp[o], J.b[o], J.b1[o], J.ca[o], J.cl[o], J.cm1[o]};
+            const unsigned o = (unsigned)i * rowb + xb;
+            ixx = gld(xx0, o); izz = gld(zz0, o); ixz = gld(xz0, o); iu = gld(uu0, o); iw = gld(ww0, o);
+            cr = CoefRow<T>{gld(ctemp, o), gld(cb, o), gld(cb1, o), gld(cca, o), gld(ccl, o), gld(ccm1, o)};
         }
 #pragma unroll
         for (int k = NT; k > 0; --k) ring[k] = ring[k - 1];
@@ -106,54 +156,48 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
 #pragma unroll
         for (int l = 0; l < NT; ++l) {
             // level l+1 consumes level-l row q = i - l, produces row r = q - 1
-            const int q = i - l, r = q - 1;
+            const int r = i - l - 1;
             const CoefRow<T>& cq = ring[l];
             const CoefRow<T>& cr_ = ring[l + 1];
-            const bool qin = xin && q >= 1 && q <= nzbc - 2;
-            const bool rin = xin && r >= 1 && r <= nzbc - 2;
             // uu_new(q)
             const T xx_r = tb_right(ixx);
-            const T u_q = qin ? cq.temp * iu + cq.b * (xx_r - ixx + ixz - cxz[l]) : T(0);
+            const T u_q = cell_vel(cq.temp, iu, cq.b, xx_r - ixx + ixz - cxz[l]);
             // ww_new(r)
             const T xz_l = tb_left(cxz[l]);
-            T w_r = rin ? cr_.temp * cwin[l] + cr_.b1 * (cxz[l] - xz_l + izz - czz[l]) : T(0);
+            T w_r = cell_vel(cr_.temp, cwin[l], cr_.b1, cxz[l] - xz_l + izz - czz[l]);
             if (src5 && r == sz && x == sx) w_r = sv[l];
             // stress(r)
             const T u_r = cuo[l];
             const T uu_l = tb_left(u_r);
             const T ww_rt = tb_right(w_r);
-            T oxx = T(0), ozz = T(0), oxz = T(0);
             const T fux = u_r - uu_l;
             const T fuz = u_q - u_r;
             const T bwx = ww_rt - w_r;
             const T bwz = w_r - cwo[l];
-            if (rin) {
-                const T tc = cr_.temp;
-                oxx = tc * cxx[l] + (cr_.ca * fux + cr_.cl * bwz) * dtx;
-                ozz = (r == pt) ? T(0) : tc * czz[l] + (cr_.ca * bwz + cr_.cl * fux) * dtx;
-                oxz = tc * cxz[l] + (cr_.cm1 * (fuz + bwx)) * dtx;
-            }
+            const T tc = cr_.temp;
+            const T oxx = cell_stress2(tc, cxx[l], cr_.ca, fux, cr_.cl, bwz, dtx);
+            const T ozz = (r == pt) ? T(0) : cell_stress2(tc, czz[l], cr_.ca, bwz, cr_.cl, fux, dtx);
+            const T oxz = cell_shear(tc, cxz[l], cr_.cm1, fuz + bwx, dtx);
             // records of level l+1 (step it0 + l) owned by this wavefront
             if (out && valid[l] && r >= z0 && r < z1) {
                 const int itl = it0 + l;
                 if (r == gz && gi >= 0) {
-                    const size_t ti = (size_t)itl * J.ng + gi;
-                    J.trace_w[ti] = w_r;
-                    if (J.trace_u) J.trace_u[ti] = u_r;
+                    const size_t ti = (size_t)itl * ng + gi;
+                    trace_w[ti] = w_r;
+                    if (trace_u) trace_u[ti] = u_r;
                 }
-                if (J.nsf > 0 && (itl % J.in_wf) == 0 && xint && r > pt && r <= pt + J.nz) {
-                    const size_t plane = (size_t)J.nz * J.nx;
-                    const size_t so = (size_t)(itl / J.in_wf) * J.nsf * plane + (size_t)(r - pt - 1) * J.nx +
-                                      (x - J.nbc);
-                    if (J.nsf == 3) {
-                        J.snap[so] = fux;
-                        J.snap[so + plane] = bwz;
-                        J.snap[so + 2 * plane] = fuz + bwx;
+                if (nsf > 0 && (itl % in_wf) == 0 && xint && r > pt && r <= pt + nzi) {
+                    const size_t plane = (size_t)nzi * nxi;
+                    const size_t so = (size_t)(itl / in_wf) * nsf * plane + (size_t)(r - pt - 1) * nxi + (x - nbc);
+                    if (nsf == 3) {
+                        snap[so] = fux;
+                        snap[so + plane] = bwz;
+                        snap[so + 2 * plane] = fuz + bwx;
                     } else {
-                        J.snap[so] = fux;
-                        J.snap[so + plane] = fuz;
-                        J.snap[so + 2 * plane] = bwx;
-                        J.snap[so + 3 * plane] = bwz;
+                        snap[so] = fux;
+                        snap[so + plane] = fuz;
+                        snap[so + 2 * plane] = bwx;
+                        snap[so + 3 * plane] = bwz;
                     }
                 }
             }
@@ -167,12 +211,12 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
         // final level: row i - NT
         const int rf = i - NT;
         if (out && rf >= z0 && rf < z1) {
-            const size_t o = (size_t)rf * P + xs;
-            J.xx[wr][o] = ixx;
-            J.zz[wr][o] = izz;
-            J.xz[wr][o] = ixz;
-            J.uu[wr][o] = iu;
-            J.ww[wr][o] = iw;
+            const unsigned o = (unsigned)rf * rowb + xb;
+            gst(xx1, o, ixx);
+            gst(zz1, o, izz);
+            gst(xz1, o, ixz);
+            gst(uu1, o, iu);
+            gst(ww1, o, iw);
         }
     }
 }
@@ -192,7 +236,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
     const int lane = threadIdx.x & (kMarchLanes - 1);
     const int tile = bm.tile0 + (threadIdx.x >> 6);
     if (tile >= J.tiles_x * J.tiles_z) return;
-    const int rd = (launch - J.launch0) & 1, wr = rd ^ 1;
+    const int rd = (launch - J.launch0) & 1;
     const int tix = tile % J.tiles_x, tiz = tile / J.tiles_x;
     const int nzbc = J.nzbc, nxbc = J.nxbc;
     const size_t P = (size_t)J.pitch;
@@ -204,13 +248,26 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
     const int z0 = 1 + tiz * J.tz;
     const int z1 = min(z0 + J.tz, nzbc - 1);
 
-    const T* __restrict__ uu0 = J.uu[rd];
-    const T* __restrict__ ww0 = J.ww[rd];
-    const T* __restrict__ xx0 = J.xx[rd];
-    const T* __restrict__ zz0 = J.zz[rd];
-    const T* __restrict__ xz0 = J.xz[rd];
+    // uniform selections keep the pointers in scalar registers
+    const gptr<const T> uu0 = (gptr<const T>)(rd ? J.uu[1] : J.uu[0]);
+    const gptr<const T> ww0 = (gptr<const T>)(rd ? J.ww[1] : J.ww[0]);
+    const gptr<const T> xx0 = (gptr<const T>)(rd ? J.xx[1] : J.xx[0]);
+    const gptr<const T> zz0 = (gptr<const T>)(rd ? J.zz[1] : J.zz[0]);
+    const gptr<const T> xz0 = (gptr<const T>)(rd ? J.xz[1] : J.xz[0]);
+    const gptr<T> uu1 = (gptr<T>)(rd ? J.uu[0] : J.uu[1]);
+    const gptr<T> ww1 = (gptr<T>)(rd ? J.ww[0] : J.ww[1]);
+    const gptr<T> xx1 = (gptr<T>)(rd ? J.xx[0] : J.xx[1]);
+    const gptr<T> zz1 = (gptr<T>)(rd ? J.zz[0] : J.zz[1]);
+    const gptr<T> xz1 = (gptr<T>)(rd ? J.xz[0] : J.xz[1]);
+    const gptr<const T> ctemp = (gptr<const T>)J.temp, cb = (gptr<const T>)J.b, cb1 = (gptr<const T>)J.b1;
+    const gptr<const T> cca = (gptr<const T>)J.ca, ccl = (gptr<const T>)J.cl, ccm1 = (gptr<const T>)J.cm1;
+    const unsigned rowb = (unsigned)(P * sizeof(T));
+    const unsigned xb = (unsigned)(xs * sizeof(T));
     const T dtx = J.dtx;
     const int pt = J.pt, gz = J.gz;
+    const int in_wf = J.in_wf, nsf = J.nsf, nzi = J.nz, nxi = J.nx, nbc = J.nbc;
+    const T* const snap = J.snap;
+    double* const img = J.img;
 
     int gi = -1;
     {
@@ -243,9 +300,9 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
         T ixx = T(0), izz = T(0), ixz = T(0), iu = T(0), iw = T(0);
         CoefRow<T> cr{T(0), T(0), T(0), T(0), T(0), T(0)};
         if (i >= 0 && i < nzbc && xload) {
-            const size_t o = (size_t)i * P + xs;
-            ixx = xx0[o]; izz = zz0[o]; ixz = xz0[o]; iu = uu0[o]; iw = ww0[o];
-            cr = CoefRow<T>{J.temp[o], J.b[o], J.b1[o], J.ca[o], J.cl[o], J.cm1[o]};
+            const unsigned o = (unsigned)i * rowb + xb;
+            ixx = gld(xx0, o); izz = gld(zz0, o); ixz = gld(xz0, o); iu = gld(uu0, o); iw = gld(ww0, o);
+            cr = CoefRow<T>{gld(ctemp, o), gld(cb, o), gld(cb1, o), gld(cca, o), gld(ccl, o), gld(ccm1, o)};
         }
 #pragma unroll
         for (int k = NT; k > 0; --k) ring[k] = ring[k - 1];
@@ -253,47 +310,40 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
 
 #pragma unroll
         for (int l = 0; l < NT; ++l) {
-            const int q = i - l, r = q - 1;
+            const int r = i - l - 1;
             const CoefRow<T>& cq = ring[l];
             const CoefRow<T>& cr_ = ring[l + 1];
-            const bool qin = xin && q >= 1 && q <= nzbc - 2;
-            const bool rin = xin && r >= 1 && r <= nzbc - 2;
-            // coefficient products of input row q (zero where the lane is outside the grid)
+            // coefficient products of input row q = r + 1
             const T caxx = cq.ca * ixx, clzz = cq.cl * izz, clxx = cq.cl * ixx, cazz = cq.ca * izz,
                     cmxz = cq.cm1 * ixz;
             // uu_b(q)
             const T caxx_l = tb_left(caxx), clzz_l = tb_left(clzz);
-            const T u_q = qin ? cq.temp * iu - cq.b * ((caxx - caxx_l) + (clzz - clzz_l) + (cmxz - ccmxz[l])) : T(0);
+            const T u_q = cell_adj_vel(cq.temp, iu, cq.b, (caxx - caxx_l) + (clzz - clzz_l) + (cmxz - ccmxz[l]));
             // ww_b(r) (+ adjoint source at the receiver row)
             const T cmxz_r = tb_right(ccmxz[l]);
-            T w_r = rin ? cr_.temp * cwin[l] -
-                              cr_.b1 * ((clxx - cclxx[l]) + (cazz - ccazz[l]) + (cmxz_r - ccmxz[l]))
-                        : T(0);
-            if (rin && r == gz) w_r += inj[l];
+            T w_r = cell_adj_vel(cr_.temp, cwin[l], cr_.b1, (clxx - cclxx[l]) + (cazz - ccazz[l]) + (cmxz_r - ccmxz[l]));
+            if (r == gz) w_r += inj[l];
             // adjoint stress(r)
             const T u_r = cuo[l];
             const T u_rt = tb_right(u_r);
             const T w_l = tb_left(w_r);
-            T oxx = T(0), ozz = T(0), oxz = T(0);
-            if (rin) {
-                const T tc = cr_.temp;
-                oxx = tc * cxx[l] - dtx * (u_rt - u_r);
-                ozz = (r == pt) ? T(0) : tc * czz[l] - dtx * (w_r - cwo[l]);
-                oxz = tc * cxz[l] - dtx * (u_q - u_r) - dtx * (w_r - w_l);
-            }
-            if (out && valid[l] && r >= z0 && r < z1 && (itl[l] % J.in_wf) == 0 && xint && r > pt &&
-                r <= pt + J.nz) {
-                const size_t plane = (size_t)J.nz * J.nx;
-                const size_t io = (size_t)(r - pt - 1) * J.nx + (x - J.nbc);
-                const T* sn = J.snap + (size_t)(itl[l] / J.in_wf) * J.nsf * plane + io;
+            const T tc = cr_.temp;
+            const T oxx = cell_adj_stress(tc, cxx[l], dtx, u_rt - u_r);
+            const T ozz = (r == pt) ? T(0) : cell_adj_stress(tc, czz[l], dtx, w_r - cwo[l]);
+            const T oxz = cell_adj_shear(tc, cxz[l], dtx, u_q - u_r, w_r - w_l);
+            if (out && valid[l] && r >= z0 && r < z1 && (itl[l] % in_wf) == 0 && xint && r > pt &&
+                r <= pt + nzi) {
+                const size_t plane = (size_t)nzi * nxi;
+                const size_t io = (size_t)(r - pt - 1) * nxi + (x - nbc);
+                const T* sn = snap + (size_t)(itl[l] / in_wf) * nsf * plane + io;
                 double Fux, Fwz, Fsh;
-                if (J.nsf == 3) {
+                if (nsf == 3) {
                     Fux = sn[0]; Fwz = sn[plane]; Fsh = sn[2 * plane];
                 } else {
                     Fux = sn[0]; Fwz = sn[3 * plane]; Fsh = (double)(sn[plane] + sn[2 * plane]);
                 }
                 const double X = oxx, Z = ozz, XZ = oxz;
-                double* im = J.img + io;
+                double* im = img + io;
                 im[0] += (Fux + Fwz) * (X + Z);
                 im[plane] += 2.0 * (X * Fux + Z * Fwz + 0.5 * XZ * Fsh);
                 im[2 * plane] += (Fux + Fwz) * (Fux + Fwz);
@@ -306,12 +356,12 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
         }
         const int rf = i - NT;
         if (out && rf >= z0 && rf < z1) {
-            const size_t o = (size_t)rf * P + xs;
-            J.xx[wr][o] = ixx;
-            J.zz[wr][o] = izz;
-            J.xz[wr][o] = ixz;
-            J.uu[wr][o] = iu;
-            J.ww[wr][o] = iw;
+            const unsigned o = (unsigned)rf * rowb + xb;
+            gst(xx1, o, ixx);
+            gst(zz1, o, izz);
+            gst(xz1, o, ixz);
+            gst(uu1, o, iu);
+            gst(ww1, o, iw);
         }
     }
 }

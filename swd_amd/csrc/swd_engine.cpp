@@ -137,6 +137,7 @@ int Engine::set_model(const double* vp, const double* vs)
         const double* srcp = c.data() + (size_t)map[k] * n;
         for (size_t q = 0; q < n; ++q) tmp[q] = (float)srcp[q];
         pack_grid(tmp.data(), nzbc, nxbc, pitch, h.data() + (size_t)k * plane);
+        zero_ring(h.data() + (size_t)k * plane, nzbc, nxbc, pitch);
     }
     SWD_HIP(hipMemcpyAsync(coefs.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, st));
     SWD_HIP(hipMemcpyAsync(vs_d.p, vs, splane * sizeof(double), hipMemcpyHostToDevice, st));

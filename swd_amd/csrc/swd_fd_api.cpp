@@ -63,6 +63,23 @@ void pack_grid(const T* cm, int nzbc, int nxbc, int pitch, T* out)
 template void pack_grid<float>(const float*, int, int, int, float*);
 template void pack_grid<double>(const double*, int, int, int, double*);
 
+// The reference never updates the outer ring of the padded grid; with zero
+// coefficients there, the fused kernels compute that ring as exact zeros.
+template <typename T>
+void zero_ring(T* plane, int nzbc, int nxbc, int pitch)
+{
+    for (int j = 0; j < nxbc; ++j) {
+        plane[j] = T(0);
+        plane[(size_t)(nzbc - 1) * pitch + j] = T(0);
+    }
+    for (int i = 0; i < nzbc; ++i) {
+        plane[(size_t)i * pitch] = T(0);
+        plane[(size_t)i * pitch + nxbc - 1] = T(0);
+    }
+}
+template void zero_ring<float>(float*, int, int, int);
+template void zero_ring<double>(double*, int, int, int);
+
 // ---------------------------------------------------------------------------
 // decoded FD descriptors
 // ---------------------------------------------------------------------------
@@ -140,11 +157,12 @@ static int staggerfd_impl(const T* iv, const T* const coef_in[7], const T* s, T*
     DevBuf coefs, state, strain, traces, snaps, src;
     // coefficient order in the device block: temp, b, b1, ca, cl, cm1
     const int map[6] = {0, 5, 6, 1, 2, 4};
-    {
-        std::vector<T> h(6 * plane);
-        for (int k = 0; k < 6; ++k) pack_grid(coef_in[map[k]], d.nzbc, d.nxbc, P, h.data() + k * plane);
-        if ((rc = coefs.upload(h.data(), h.size() * sizeof(T), st.s))) return rc;
+    std::vector<T> hc(6 * plane);
+    for (int k = 0; k < 6; ++k) {
+        pack_grid(coef_in[map[k]], d.nzbc, d.nxbc, P, hc.data() + k * plane);
+        if (fused) zero_ring(hc.data() + k * plane, d.nzbc, d.nxbc, P);
     }
+    if ((rc = coefs.upload(hc.data(), hc.size() * sizeof(T), st.s))) return rc;
     if ((rc = state.alloc(10 * plane * sizeof(T))) || (rc = traces.alloc(2 * ntr * sizeof(T))) ||
         (rc = src.upload(s, (size_t)d.nt * sizeof(T), st.s)))
         return rc;
@@ -265,11 +283,12 @@ static int e2drtm_impl(const T* iv, const T* const coef_in[7], const T* adj, con
     if ((rc = st.create())) return rc;
     DevBuf coefs, state, dadj, snaps, img;
     const int map[6] = {0, 5, 6, 1, 2, 4};
-    {
-        std::vector<T> h(6 * plane);
-        for (int k = 0; k < 6; ++k) pack_grid(coef_in[map[k]], d.nzbc, d.nxbc, P, h.data() + k * plane);
-        if ((rc = coefs.upload(h.data(), h.size() * sizeof(T), st.s))) return rc;
+    std::vector<T> hc(6 * plane);
+    for (int k = 0; k < 6; ++k) {
+        pack_grid(coef_in[map[k]], d.nzbc, d.nxbc, P, hc.data() + k * plane);
+        if (fused) zero_ring(hc.data() + k * plane, d.nzbc, d.nxbc, P);
     }
+    if ((rc = coefs.upload(hc.data(), hc.size() * sizeof(T), st.s))) return rc;
     {
         std::vector<T> h(ntr);
         for (int it = 0; it < d.nt; ++it)

@@ -20,6 +20,8 @@ int fd_force_plain();
 
 template <typename T>
 void pack_grid(const T* colmajor, int nzbc, int nxbc, int pitch, T* out);
+template <typename T>
+void zero_ring(T* plane, int nzbc, int nxbc, int pitch);
 
 int prepare_model(const double* vp, const double* vs, int nz, int nx, int nbc, double dx, double dt, int fd_order,
                   double* temp, double* ca, double* cl, double* cm, double* cm1, double* b, double* b1);
