@@ -66,6 +66,25 @@ struct CoefRow {
     T temp, b, b1, ca, cl, cm1;
 };
 
+// One row of state and coefficients for this lane.  Rows above/below the
+// grid are clamped onto the first/last row, which (like the first/last
+// column) is the zero ring of every state and coefficient plane.
+template <typename T>
+struct RowLoad {
+    T xx, zz, xz, uu, ww;
+    CoefRow<T> c;
+    __device__ __forceinline__ void load(gptr<const T> xx0, gptr<const T> zz0, gptr<const T> xz0, gptr<const T> uu0,
+                                         gptr<const T> ww0, gptr<const T> ct, gptr<const T> cb, gptr<const T> cb1,
+                                         gptr<const T> cca, gptr<const T> ccl, gptr<const T> ccm1, int row, int nzbc,
+                                         unsigned rowb, unsigned xb)
+    {
+        const int rc = min(max(row, 0), nzbc - 1);
+        const unsigned o = (unsigned)rc * rowb + xb;
+        xx = gld(xx0, o); zz = gld(zz0, o); xz = gld(xz0, o); uu = gld(uu0, o); ww = gld(ww0, o);
+        c = CoefRow<T>{gld(ct, o), gld(cb, o), gld(cb1, o), gld(cca, o), gld(ccl, o), gld(ccm1, o)};
+    }
+};
+
 template <typename T, int NT>
 __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJob<T>* __restrict__ jobs,
                                                                       const BlockMap* __restrict__ bmap, int step,
@@ -84,10 +103,10 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
     const int nzbc = J.nzbc, nxbc = J.nxbc;
     const size_t P = (size_t)J.pitch;
     const int x = tix * W + 1 - NT + lane;
-    const bool xload = x >= 0 && x < nxbc;
     const bool xin = x >= 1 && x <= nxbc - 2;
     const bool out = xin && lane >= NT && lane < NT + W;
-    const int xs = xload ? x : 0;
+    // lanes and rows off the grid read the (all-zero) ring instead
+    const int xs = min(max(x, 0), nxbc - 1);
     const int z0 = 1 + tiz * J.tz;
     const int z1 = min(z0 + J.tz, nzbc - 1);
 
@@ -111,9 +130,9 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
     const bool src5 = J.stype == 5;
     const int nt = J.nt;
     const int ng = J.ng, in_wf = J.in_wf, nsf = J.nsf, nzi = J.nz, nxi = J.nx, nbc = J.nbc;
-    T* const trace_w = J.trace_w;
-    T* const trace_u = J.trace_u;
-    T* const snap = J.snap;
+    const gptr<T> trace_w = (gptr<T>)J.trace_w;
+    const gptr<T> trace_u = (gptr<T>)J.trace_u;
+    const gptr<T> snap = (gptr<T>)J.snap;
 
     // per-level step, source value and validity
     T sv[NT];
@@ -140,18 +159,14 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
     for (int k = 0; k <= NT; ++k) ring[k] = CoefRow<T>{T(0), T(0), T(0), T(0), T(0), T(0)};
 
     const int i0 = z0 - NT - 1, i1 = z1 - 1 + NT;
-    for (int i = i0; i <= i1; ++i) {
-        // level 0: row i of the stored state and its coefficients (0 off the grid)
-        T ixx = T(0), izz = T(0), ixz = T(0), iu = T(0), iw = T(0);
-        CoefRow<T> cr{T(0), T(0), T(0), T(0), T(0), T(0)};
-        if (i >= 0 && i < nzbc && xload) {
-            const unsigned o = (unsigned)i * rowb + xb;
-            ixx = gld(xx0, o); izz = gld(zz0, o); ixz = gld(xz0, o); iu = gld(uu0, o); iw = gld(ww0, o);
-            cr = CoefRow<T>{gld(ctemp, o), gld(cb, o), gld(cb1, o), gld(cca, o), gld(ccl, o), gld(ccm1, o)};
-        }
+    // row loads are issued one iteration ahead so their latency overlaps the
+    // level updates of the current row
+    auto row = [&](const RowLoad<T>& cur, const int i) {
+        // level 0: row i of the stored state and its coefficients
+        T ixx = cur.xx, izz = cur.zz, ixz = cur.xz, iu = cur.uu, iw = cur.ww;
 #pragma unroll
         for (int k = NT; k > 0; --k) ring[k] = ring[k - 1];
-        ring[0] = cr;
+        ring[0] = cur.c;
 
 #pragma unroll
         for (int l = 0; l < NT; ++l) {
@@ -218,6 +233,17 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
             gst(uu1, o, iu);
             gst(ww1, o, iw);
         }
+    };
+    // Rows are loaded one step ahead into two alternating buffers, so the
+    // loads of row i+1 are in flight while row i is processed.
+    RowLoad<T> A, B;
+    A.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i0, nzbc, rowb, xb);
+    for (int i = i0; i <= i1; i += 2) {
+        B.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i + 1, nzbc, rowb, xb);
+        row(A, i);
+        if (i + 1 > i1) break;
+        A.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i + 2, nzbc, rowb, xb);
+        row(B, i + 1);
     }
 }
 
@@ -241,10 +267,10 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
     const int nzbc = J.nzbc, nxbc = J.nxbc;
     const size_t P = (size_t)J.pitch;
     const int x = tix * W + 1 - NT + lane;
-    const bool xload = x >= 0 && x < nxbc;
     const bool xin = x >= 1 && x <= nxbc - 2;
     const bool out = xin && lane >= NT && lane < NT + W;
-    const int xs = xload ? x : 0;
+    // lanes and rows off the grid read the (all-zero) ring instead
+    const int xs = min(max(x, 0), nxbc - 1);
     const int z0 = 1 + tiz * J.tz;
     const int z1 = min(z0 + J.tz, nzbc - 1);
 
@@ -266,8 +292,8 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
     const T dtx = J.dtx;
     const int pt = J.pt, gz = J.gz;
     const int in_wf = J.in_wf, nsf = J.nsf, nzi = J.nz, nxi = J.nx, nbc = J.nbc;
-    const T* const snap = J.snap;
-    double* const img = J.img;
+    const gptr<const T> snap = (gptr<const T>)J.snap;
+    const gptr<double> img = (gptr<double>)J.img;
 
     int gi = -1;
     {
@@ -296,17 +322,11 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
     for (int k = 0; k <= NT; ++k) ring[k] = CoefRow<T>{T(0), T(0), T(0), T(0), T(0), T(0)};
 
     const int i0 = z0 - NT - 1, i1 = z1 - 1 + NT;
-    for (int i = i0; i <= i1; ++i) {
-        T ixx = T(0), izz = T(0), ixz = T(0), iu = T(0), iw = T(0);
-        CoefRow<T> cr{T(0), T(0), T(0), T(0), T(0), T(0)};
-        if (i >= 0 && i < nzbc && xload) {
-            const unsigned o = (unsigned)i * rowb + xb;
-            ixx = gld(xx0, o); izz = gld(zz0, o); ixz = gld(xz0, o); iu = gld(uu0, o); iw = gld(ww0, o);
-            cr = CoefRow<T>{gld(ctemp, o), gld(cb, o), gld(cb1, o), gld(cca, o), gld(ccl, o), gld(ccm1, o)};
-        }
+    auto row = [&](const RowLoad<T>& cur, const int i) {
+        T ixx = cur.xx, izz = cur.zz, ixz = cur.xz, iu = cur.uu, iw = cur.ww;
 #pragma unroll
         for (int k = NT; k > 0; --k) ring[k] = ring[k - 1];
-        ring[0] = cr;
+        ring[0] = cur.c;
 
 #pragma unroll
         for (int l = 0; l < NT; ++l) {
@@ -335,7 +355,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
                 r <= pt + nzi) {
                 const size_t plane = (size_t)nzi * nxi;
                 const size_t io = (size_t)(r - pt - 1) * nxi + (x - nbc);
-                const T* sn = snap + (size_t)(itl[l] / in_wf) * nsf * plane + io;
+                const gptr<const T> sn = snap + (size_t)(itl[l] / in_wf) * nsf * plane + io;
                 double Fux, Fwz, Fsh;
                 if (nsf == 3) {
                     Fux = sn[0]; Fwz = sn[plane]; Fsh = sn[2 * plane];
@@ -343,7 +363,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
                     Fux = sn[0]; Fwz = sn[3 * plane]; Fsh = (double)(sn[plane] + sn[2 * plane]);
                 }
                 const double X = oxx, Z = ozz, XZ = oxz;
-                double* im = img + io;
+                const gptr<double> im = img + io;
                 im[0] += (Fux + Fwz) * (X + Z);
                 im[plane] += 2.0 * (X * Fux + Z * Fwz + 0.5 * XZ * Fsh);
                 im[2 * plane] += (Fux + Fwz) * (Fux + Fwz);
@@ -363,6 +383,17 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
             gst(uu1, o, iu);
             gst(ww1, o, iw);
         }
+    };
+    // Rows are loaded one step ahead into two alternating buffers, so the
+    // loads of row i+1 are in flight while row i is processed.
+    RowLoad<T> A, B;
+    A.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i0, nzbc, rowb, xb);
+    for (int i = i0; i <= i1; i += 2) {
+        B.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i + 1, nzbc, rowb, xb);
+        row(A, i);
+        if (i + 1 > i1) break;
+        A.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i + 2, nzbc, rowb, xb);
+        row(B, i + 1);
     }
 }
 
