@@ -89,8 +89,9 @@ def main():
     ap.add_argument("--shots", type=int, default=60)
     ap.add_argument("--nt", type=int, default=6000)
     ap.add_argument("--tile-rows", type=int, default=0)
-    ap.add_argument("--time-block", type=int, default=4, help="FD steps per launch (1: single-step kernels)")
-    ap.add_argument("--band-rows", type=int, default=32, help="band rows of the time-blocked kernels")
+    ap.add_argument("--time-block", type=int, default=0,
+                    help="FD steps per launch (1: single-step kernels; 0: chosen by the autotuner)")
+    ap.add_argument("--band-rows", type=int, default=0, help="band rows of the time-blocked kernels (0: automatic)")
     ap.add_argument("--plain-shots", type=int, default=2, help="shots for the unfused comparator (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -106,14 +107,21 @@ def main():
     eng = Engine(prob)
     if args.tile_rows:
         eng.set_tile_rows(args.tile_rows)
-    eng.set_time_block(args.time_block, args.band_rows)
     eng.set_wavelet(mod["s"])
     eng.set_model(mod["vp_true"], mod["vs_true"])
     eng.observe()                          # observed picks cr_0 / cr_0l (untimed, SWD.m:150-162)
     eng.set_model(mod["vp"], mod["vs"])
+    tuned = None
+    if args.time_block > 0:
+        eng.set_time_block(args.time_block, args.band_rows)
+    else:                                  # untimed: the autotuner picks the FD launch shape
+        t0 = time.perf_counter()
+        tune = eng.autotune()
+        tuned = {"fwd": list(tune["fwd"]), "adj": list(tune["adj"]), "candidates": tune["candidates"],
+                 "seconds": round(time.perf_counter() - t0, 2)}
     for _ in range(args.warmup):
         eng.gradient()
-    per_step, fwd_us, adj_us, stage, spl = [], [], [], [], []
+    per_step, fwd_us, adj_us, stage = [], [], [], []
     t_all = time.perf_counter()
     for _ in range(args.steps):
         t0 = time.perf_counter()
@@ -122,7 +130,6 @@ def main():
         t = eng.timings()
         fwd_us.append(t["fwd_kernel_us"])
         adj_us.append(t["adj_kernel_us"])
-        spl.append(int(t["steps_per_launch"]))
         stage.append(t)
     wall = time.perf_counter() - t_all
     info = eng.info()
@@ -135,12 +142,13 @@ def main():
     # roofline of the dominant kernel (forward or adjoint march), event-timed in the run;
     # one launch advances `k` time steps, so its algorithmic bytes are k steps' worth
     f_us, a_us = float(np.mean(fwd_us)), float(np.mean(adj_us))
-    k = spl[-1]
-    kind = "tb" if k > 1 else "march"
-    if f_us * nt >= a_us * (nt - 1):
-        kname, kus, kbytes = f"fd_fwd_{kind}", f_us, k * fd_bytes_per_launch(prob, S, False)
+    kf, ka = int(stage[-1]["fwd_steps_per_launch"]), int(stage[-1]["adj_steps_per_launch"])
+    if f_us * nt / kf >= a_us * (nt - 1) / ka:
+        k = kf
+        kname, kus, kbytes = f"fd_fwd_{'tb' if k > 1 else 'march'}", f_us, k * fd_bytes_per_launch(prob, S, False)
     else:
-        kname, kus, kbytes = f"fd_adj_{kind}", a_us, k * fd_bytes_per_launch(prob, S, True)
+        k = ka
+        kname, kus, kbytes = f"fd_adj_{'tb' if k > 1 else 'march'}", a_us, k * fd_bytes_per_launch(prob, S, True)
     achieved = kbytes / (kus * 1e-6) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": kname,
@@ -160,6 +168,8 @@ def main():
            "stage_ms": {k: round(float(np.mean([s[k] for s in stage])), 2)
                         for k in ("fwd_ms", "disp_ms", "adj_ms", "asm_ms")},
            "roofline": roof,
+           "fd_launch": {"fwd": [kf, int(stage[-1]["fwd_band_rows"])], "adj": [ka, int(stage[-1]["adj_band_rows"])],
+                         "autotune": tuned},
            "device_bytes": info["device_bytes"],
            "residual_r_mean": float(np.mean(g["res_r"]))}
 

@@ -80,8 +80,10 @@ int swd_e2drtm_f64(const double *iv, const double *temp, const double *ca, const
 int swd_set_fd_path(int mode);
 
 /* Time blocking of the fused FD kernels: nt_block steps per launch (1..6;
- * 1 = one launch per time step) and band rows of the time-blocked kernels.
- * Results do not depend on either setting (bitwise). */
+ * 1 = one launch per time step) and band rows of the time-blocked kernels
+ * (0 = chosen from the kernels' occupancy and the grid, the default).
+ * Results do not depend on either setting (bitwise).  No reference
+ * counterpart: the MEX time loops have no launch structure. */
 int swd_set_fd_time_block(int nt_block, int rows);
 
 /*
@@ -197,12 +199,27 @@ int swd_engine_set_fd_path(swd_engine *e, int mode);
 int swd_engine_run(swd_engine *e, int stage);
 int swd_engine_set_picks(swd_engine *e, const double *cr_r, const double *cr_l);
 int swd_engine_set_tile_rows(swd_engine *e, int tz);
+/* Time block of both FD phases (rows 0 = automatic), or of one phase
+ * (phase 0 forward, 1 adjoint, -1 both). */
 int swd_engine_set_time_block(swd_engine *e, int nt_block, int rows);
+int swd_engine_set_phase_time_block(swd_engine *e, int phase, int nt_block, int rows);
+/* FD autotuner (no reference counterpart): times every time block 1..6 with
+ * band heights that split the tallest grid evenly, over `steps` (<= 0: 60)
+ * forward and adjoint steps of the batch on its own buffers, and installs the
+ * fastest forward and adjoint settings on every engine.  choice[4] =
+ * {forward steps/launch, forward band rows, adjoint steps/launch, adjoint
+ * band rows}; table[table_rows][4] = {steps/launch, rows, forward us/step,
+ * adjoint us/step} per candidate; *n_rows = candidates timed.  Overwrites FD
+ * state, gathers, snapshots, adjoint sources and images. */
+int swd_batch_autotune(swd_engine **engines, int n, int steps, int *choice, double *table, int table_rows,
+                       int *n_rows);
+int swd_engine_autotune(swd_engine *e, int steps, int *choice, double *table, int table_rows, int *n_rows);
 /* vals[0..9] = nzbc, nxbc, pitch, nt/in_wf, npair, np, device bytes, ccn, lf, tile rows */
 int swd_engine_info(swd_engine *e, long long *vals, int n);
-/* ms[0..6] = forward, dispersion, adjoint, assembly (ms) of the last gradient,
+/* ms[0..9] = forward, dispersion, adjoint, assembly (ms) of the last gradient,
  * mean forward / adjoint march-kernel launch duration (us, event-timed
- * sample), time steps advanced per sampled launch */
+ * sample), forward / adjoint time steps per launch, forward / adjoint band
+ * rows */
 int swd_engine_timings(swd_engine *e, double *ms, int n);
 /* Copy a device buffer to host: 0 gathers [shot][t][g] f32, 1 adjoint sources
  * [shot][t][g] f32, 2 {dk_vs, sum cl_img, sum cm_img, sum illum} nz x nx f64,

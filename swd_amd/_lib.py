@@ -60,6 +60,9 @@ SIGNATURES = {
     "swd_engine_set_picks": (_I, [_P, _P, _P]),
     "swd_engine_set_tile_rows": (_I, [_P, _I]),
     "swd_engine_set_time_block": (_I, [_P, _I, _I]),
+    "swd_engine_set_phase_time_block": (_I, [_P, _I, _I, _I]),
+    "swd_engine_autotune": (_I, [_P, _I, _P, _P, _I, _P]),
+    "swd_batch_autotune": (_I, [_P, _I, _I, _P, _P, _I, _P]),
     "swd_engine_info": (_I, [_P, _P, _I]),
     "swd_engine_timings": (_I, [_P, _P, _I]),
     "swd_engine_read": (_I, [_P, _I, _P, ctypes.c_longlong]),

@@ -431,6 +431,29 @@ int launch_adj_tb(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap
     return kOk;
 }
 
+// Resident workgroups per CU of the time-blocked kernel (occupancy API), 0 if unknown.
+template <typename T>
+int tb_blocks_per_cu(bool adj, int nt_block)
+{
+    const void* f = nullptr;
+    switch (nt_block) {
+    case 2: f = adj ? (const void*)fd_adj_tb<T, 2> : (const void*)fd_fwd_tb<T, 2>; break;
+    case 3: f = adj ? (const void*)fd_adj_tb<T, 3> : (const void*)fd_fwd_tb<T, 3>; break;
+    case 4: f = adj ? (const void*)fd_adj_tb<T, 4> : (const void*)fd_fwd_tb<T, 4>; break;
+    case 5: f = adj ? (const void*)fd_adj_tb<T, 5> : (const void*)fd_fwd_tb<T, 5>; break;
+    case 6: f = adj ? (const void*)fd_adj_tb<T, 6> : (const void*)fd_fwd_tb<T, 6>; break;
+    default: return 0;
+    }
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kMarchLanes * kMarchWaves, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return nb;
+}
+template int tb_blocks_per_cu<float>(bool, int);
+template int tb_blocks_per_cu<double>(bool, int);
+
 template int launch_fwd_tb<float>(hipStream_t, const FdJob<float>*, const BlockMap*, int, int, int, int);
 template int launch_adj_tb<float>(hipStream_t, const FdJob<float>*, const BlockMap*, int, int, int, int);
 template int launch_fwd_tb<double>(hipStream_t, const FdJob<double>*, const BlockMap*, int, int, int, int);

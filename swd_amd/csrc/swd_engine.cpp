@@ -199,7 +199,7 @@ void Engine::make_jobs(std::vector<FdJob<float>>& jobs, bool adj, bool snapshots
 }
 
 int run_fd_batch(const std::vector<Engine*>& engines, bool adj, bool snapshots, hipStream_t st, int tz,
-                 float* kernel_us)
+                 MarchStats* stats)
 {
     std::vector<FdJob<float>> jobs;
     int steps = 0;
@@ -211,20 +211,24 @@ int run_fd_batch(const std::vector<Engine*>& engines, bool adj, bool snapshots, 
         else SWD_HIP(hipMemsetAsync(e->traces.p, 0, e->traces.bytes, st));
     }
     const Engine* e0 = engines[0];
-    int nt_block = 1;
-    const int rc = run_march<float>(st, jobs, adj, steps, e0->nt_block, e0->tb_rows, tz, kernel_us, &nt_block);
+    MarchStats ms;
+    const int rc = run_march<float>(st, jobs, adj, steps, e0->nt_block[adj], e0->tb_rows[adj], tz, &ms);
     if (rc == kOk)
-        for (Engine* e : engines) e->times.steps_per_launch = nt_block;
+        for (Engine* e : engines) {
+            if (adj) { e->times.adj_spl = ms.nt_block; e->times.adj_rows = ms.rows; }
+            else { e->times.fwd_spl = ms.nt_block; e->times.fwd_rows = ms.rows; }
+        }
+    if (stats) *stats = ms;
     return rc;
 }
 
 int Engine::run_fd(bool adj, bool snapshots)
 {
     if (fd_path == 0) {
-        float us = 0;
-        const int rc = run_fd_batch({this}, adj, snapshots, st, tz, &us);
-        if (adj) times.adj_kernel_us = us;
-        else times.fwd_kernel_us = us;
+        MarchStats ms;
+        const int rc = run_fd_batch({this}, adj, snapshots, st, tz, &ms);
+        if (adj) times.adj_kernel_us = ms.kernel_us;
+        else times.fwd_kernel_us = ms.kernel_us;
         return rc;
     }
     // plain comparator: one shot at a time, one launch per reference statement
@@ -450,10 +454,51 @@ int swd_engine_set_picks(swd_engine* e, const double* cr_r, const double* cr_l)
 
 int swd_engine_set_time_block(swd_engine* e, int nt_block, int rows)
 {
-    if (!e || nt_block < 1 || nt_block > 6 || rows < 1) { set_error("time block must be 1..6 steps, rows >= 1"); return kErrArg; }
-    e->e.nt_block = nt_block;
-    e->e.tb_rows = rows;
+    return swd_engine_set_phase_time_block(e, -1, nt_block, rows);
+}
+
+int swd_engine_set_phase_time_block(swd_engine* e, int phase, int nt_block, int rows)
+{
+    if (!e || nt_block < 1 || nt_block > 6 || rows < 0 || phase < -1 || phase > 1) {
+        set_error("time block must be 1..6 steps, rows >= 0 (0: automatic), phase -1..1");
+        return kErrArg;
+    }
+    for (int ph = 0; ph < 2; ++ph)
+        if (phase < 0 || phase == ph) {
+            e->e.nt_block[ph] = nt_block;
+            e->e.tb_rows[ph] = rows;
+        }
     return kOk;
+}
+
+int swd_batch_autotune(swd_engine** es, int n, int steps, int* choice, double* table, int table_rows, int* n_rows)
+{
+    if (!es || n < 1) { set_error("empty batch"); return kErrArg; }
+    std::vector<Engine*> v;
+    for (int i = 0; i < n; ++i) {
+        if (!es[i]) { set_error("null engine %d", i); return kErrArg; }
+        if (!es[i]->e.have_model || !es[i]->e.have_wavelet) { set_error("engine %d lacks model/wavelet", i); return kErrArg; }
+        v.push_back(&es[i]->e);
+    }
+    std::vector<TuneRow> rows;
+    const int rc = autotune_fd_batch(v, steps, rows);
+    if (rc) return rc;
+    if (choice) {
+        choice[0] = v[0]->nt_block[0]; choice[1] = v[0]->tb_rows[0];
+        choice[2] = v[0]->nt_block[1]; choice[3] = v[0]->tb_rows[1];
+    }
+    if (table)
+        for (int i = 0; i < (int)rows.size() && i < table_rows; ++i) {
+            table[4 * i + 0] = rows[i].nt_block; table[4 * i + 1] = rows[i].rows;
+            table[4 * i + 2] = rows[i].fwd_us; table[4 * i + 3] = rows[i].adj_us;
+        }
+    if (n_rows) *n_rows = (int)rows.size();
+    return kOk;
+}
+
+int swd_engine_autotune(swd_engine* e, int steps, int* choice, double* table, int table_rows, int* n_rows)
+{
+    return swd_batch_autotune(&e, 1, steps, choice, table, table_rows, n_rows);
 }
 
 int swd_engine_info(swd_engine* e, long long* vals, int n)
@@ -470,9 +515,9 @@ int swd_engine_timings(swd_engine* e, double* ms, int n)
 {
     if (!e || !ms) return kErrArg;
     const StageTimes& t = e->e.times;
-    const double v[7] = {t.fwd_ms, t.disp_ms, t.adj_ms, t.asm_ms, t.fwd_kernel_us, t.adj_kernel_us,
-                         (double)t.steps_per_launch};
-    for (int i = 0; i < n && i < 7; ++i) ms[i] = v[i];
+    const double v[10] = {t.fwd_ms, t.disp_ms, t.adj_ms, t.asm_ms, t.fwd_kernel_us, t.adj_kernel_us,
+                          (double)t.fwd_spl, (double)t.adj_spl, (double)t.fwd_rows, (double)t.adj_rows};
+    for (int i = 0; i < n && i < 10; ++i) ms[i] = v[i];
     return kOk;
 }
 
@@ -508,7 +553,7 @@ int swd_batch_gradient(swd_engine** es, int n, double** dk_vs, double* kernel_us
     hipStream_t st = v[0]->st;
     // every engine's stream must be idle before the shared launches
     for (Engine* x : v) SWD_HIP(hipStreamSynchronize(x->st));
-    float us[2] = {0, 0};
+    MarchStats us[2];
     int rc = run_fd_batch(v, false, true, st, v[0]->tz, &us[0]);
     if (rc) return rc;
     for (Engine* x : v) {
@@ -523,7 +568,7 @@ int swd_batch_gradient(swd_engine** es, int n, double** dk_vs, double* kernel_us
             SWD_HIP(hipMemcpyAsync(dk_vs[i], x->outs.p, x->splane * sizeof(double), hipMemcpyDeviceToHost, x->st));
         SWD_HIP(hipStreamSynchronize(x->st));
     }
-    if (kernel_us) { kernel_us[0] = us[0]; kernel_us[1] = us[1]; }
+    if (kernel_us) { kernel_us[0] = us[0].kernel_us; kernel_us[1] = us[1].kernel_us; }
     return kOk;
 }
 

@@ -18,8 +18,8 @@ int launch_assemble_gradient(hipStream_t st, const double* imgs, int nshots, int
 struct StageTimes {
     float fwd_ms = 0, disp_ms = 0, adj_ms = 0, asm_ms = 0;
     float fwd_kernel_us = 0, adj_kernel_us = 0;  // mean duration of one march launch (sampled)
-    int steps_per_launch = 1;                    // time steps advanced by one sampled launch
-    int fwd_samples = 0, adj_samples = 0;
+    int fwd_spl = 1, adj_spl = 1;                // time steps advanced by one sampled launch
+    int fwd_rows = 0, adj_rows = 0;              // band rows used
 };
 
 class Engine {
@@ -32,8 +32,8 @@ public:
     bool own_stream = false;
     int fd_path = 0;  // 0 fused batched, 1 plain one-shot-per-launch
     int tz = 16;        // band rows of the single-step march
-    int nt_block = 4;   // time steps per time-blocked launch (1: single-step kernels only)
-    int tb_rows = 32;   // band rows of the time-blocked march
+    int nt_block[2] = {4, 4};  // time steps per launch, forward / adjoint (1: single-step kernels only)
+    int tb_rows[2] = {0, 0};   // band rows of the time-blocked march, forward / adjoint (0: from occupancy)
     bool have_model = false, have_wavelet = false;
     double vpmax = 0;
 
@@ -77,10 +77,24 @@ private:
     int run_fd(bool adjoint, bool snapshots);
 };
 
-// Run the FD phase of several engines in shared launches (one launch per
-// time step covers every shot of every engine; engines may differ in grid
-// size and nt).  adjoint selects the reverse-time phase.
+// Run the FD phase of several engines in shared launches (one launch covers
+// every shot of every engine; engines may differ in grid size and nt) with
+// the first engine's time-block settings.  adjoint selects the reverse-time
+// phase.
 int run_fd_batch(const std::vector<Engine*>& engines, bool adjoint, bool snapshots, hipStream_t st, int tz,
-                 float* kernel_us);
+                 MarchStats* stats);
+
+// One measured candidate of the FD autotuner.
+struct TuneRow {
+    int nt_block = 1, rows = 0;
+    float fwd_us = 0, adj_us = 0;  // event-timed microseconds per time step over the batch
+};
+
+// Measure the FD march of a batch of engines for every time block 1..6 and a
+// set of band heights, on the engines' own buffers over `steps` steps of each
+// phase, and install the fastest forward and adjoint settings on every
+// engine.  Overwrites the FD state, gathers, snapshots, adjoint sources and
+// images (the next gradient recomputes all of them).
+int autotune_fd_batch(const std::vector<Engine*>& engines, int steps, std::vector<TuneRow>& table);
 
 }  // namespace swd

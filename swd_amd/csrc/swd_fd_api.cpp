@@ -195,8 +195,7 @@ static int staggerfd_impl(const T* iv, const T* const coef_in[7], const T* s, T*
     J.dtx = (T)d.dtx; J.dt = T(0);
     if (fused) {
         std::vector<FdJob<T>> jobs{J};
-        if ((rc = run_march<T>(st.s, jobs, false, d.nt, fd_time_block(), fd_tb_rows(), fd_tile_rows(), nullptr,
-                               nullptr)))
+        if ((rc = run_march<T>(st.s, jobs, false, d.nt, fd_time_block(), fd_tb_rows(), fd_tile_rows(), nullptr)))
             return rc;
     } else {
         if ((rc = strain.alloc(4 * plane * sizeof(T)))) return rc;
@@ -332,8 +331,7 @@ static int e2drtm_impl(const T* iv, const T* const coef_in[7], const T* adj, con
     J.dtx = (T)d.dtx; J.dt = (T)d.dt;
     if (fused) {
         std::vector<FdJob<T>> jobs{J};
-        if ((rc = run_march<T>(st.s, jobs, true, d.nt - 1, fd_time_block(), fd_tb_rows(), fd_tile_rows(), nullptr,
-                               nullptr)))
+        if ((rc = run_march<T>(st.s, jobs, true, d.nt - 1, fd_time_block(), fd_tb_rows(), fd_tile_rows(), nullptr)))
             return rc;
     } else {
         for (int it = d.nt - 2; it >= 0; --it)
@@ -470,7 +468,10 @@ int swd_set_fd_path(int mode) { return g_fd_path.exchange(mode ? 1 : 0); }
 
 int swd_set_fd_time_block(int nt_block, int rows)
 {
-    if (nt_block < 1 || nt_block > 6 || rows < 1) { set_error("time block must be 1..6 steps, rows >= 1"); return kErrArg; }
+    if (nt_block < 1 || nt_block > 6 || rows < 0) {
+        set_error("time block must be 1..6 steps, rows >= 0 (0: automatic)");
+        return kErrArg;
+    }
     set_fd_time_block(nt_block, rows);
     return kOk;
 }

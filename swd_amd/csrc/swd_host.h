@@ -111,19 +111,35 @@ template <typename T>
 int launch_adj_tb(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step, int launch,
                   int nt_block);
 
+template <typename T>
+int tb_blocks_per_cu(bool adjoint, int nt_block);
+
 // Time-blocking settings: steps per launch (1 = single-step kernels) and band
-// rows of the time-blocked kernels; process-wide defaults for the MEX path.
+// rows of the time-blocked kernels (0 = chosen from occupancy); process-wide
+// defaults for the MEX path.
 int fd_time_block();
 int fd_tb_rows();
 void set_fd_time_block(int nt_block, int rows);
 
+// Band rows for a time-blocked launch over `jobs` when none are given: the
+// band height that minimises (waves rounds) x (rows loaded per wave), with the
+// resident-wave count taken from the occupancy API.
+template <typename T>
+int auto_band_rows(const std::vector<FdJob<T>>& jobs, bool adjoint, int nt_block);
+
+// What one run_march call did: mean event-timed duration of one launch of the
+// dominant kernel (sampled), event-timed span of the whole launch sequence,
+// steps per time-blocked launch (1 if none) and band rows used.
+struct MarchStats {
+    float kernel_us = 0, span_ms = 0;
+    int nt_block = 1, rows = 0;
+};
+
 // Advance a batch of jobs (all starting together) over `steps` forward or
 // adjoint steps with time-blocked launches plus single-step remainders.
-// kernel_us receives the mean event-timed duration of one launch of the
-// dominant kernel, launches_tb/launches_ss the launch counts.
 template <typename T>
 int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adjoint, int steps, int nt_block, int tb_rows,
-              int ss_rows, float* kernel_us, int* launches_tb);
+              int ss_rows, MarchStats* stats);
 template <typename T>
 int plain_forward_step(hipStream_t st, const FdJob<T>& J, T* const w[4], int it);
 template <typename T>

@@ -12,14 +12,14 @@
 namespace swd {
 
 static std::atomic<int> g_nt_block{4};
-static std::atomic<int> g_tb_rows{32};
+static std::atomic<int> g_tb_rows{0};
 
 int fd_time_block() { return g_nt_block.load(); }
 int fd_tb_rows() { return g_tb_rows.load(); }
 void set_fd_time_block(int nt_block, int rows)
 {
     g_nt_block.store(std::min(std::max(nt_block, 1), 6));
-    g_tb_rows.store(std::max(rows, 1));
+    g_tb_rows.store(std::max(rows, 0));
 }
 
 template <typename T>
@@ -42,8 +42,37 @@ template int build_march_plan<float>(std::vector<FdJob<float>>&, int, MarchPlan&
 template int build_march_plan<double>(std::vector<FdJob<double>>&, int, MarchPlan&, int);
 
 template <typename T>
+int auto_band_rows(const std::vector<FdJob<T>>& jobs, bool adj, int NT)
+{
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        cus = 0;
+    }
+    const int nb = tb_blocks_per_cu<T>(adj, NT);
+    const long long slots = (long long)std::max(cus, 1) * std::max(nb, 1) * kMarchWaves;
+    const int out_cols = kMarchLanes - 2 * NT;
+    int zmax = 1;
+    for (const auto& J : jobs) zmax = std::max(zmax, J.nzbc - 2);
+    int best = zmax;
+    double best_cost = -1;
+    for (int b = 1; b <= 64; ++b) {
+        const int H = div_up(zmax, b);
+        if (H < 8 && b > 1) break;
+        long long waves = 0;
+        for (const auto& J : jobs) waves += (long long)div_up(J.nxbc - 2, out_cols) * div_up(J.nzbc - 2, H);
+        const double cost = (double)div_up((int)std::min<long long>(waves, 1LL << 30), (int)slots) * (H + 2 * NT + 1);
+        if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = H; }
+    }
+    return best;
+}
+template int auto_band_rows<float>(const std::vector<FdJob<float>>&, bool, int);
+template int auto_band_rows<double>(const std::vector<FdJob<double>>&, bool, int);
+
+template <typename T>
 int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int steps, int nt_block, int tb_rows,
-              int ss_rows, float* kernel_us, int* launches_tb)
+              int ss_rows, MarchStats* stats)
 {
     bool tb_ok = nt_block >= 2;
     for (const auto& J : jobs) {
@@ -57,6 +86,7 @@ int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int steps, 
     std::vector<FdJob<T>> jt = jobs, js = jobs;
     MarchPlan ptb, pss;
     DevBuf djt, dbt, djs, dbs;
+    if (nL > 0 && tb_rows <= 0) tb_rows = auto_band_rows(jobs, adj, NT);
     if (nL > 0) {
         if ((rc = build_march_plan(jt, tb_rows, ptb, kMarchLanes - 2 * NT))) return rc;
         if ((rc = djt.upload(jt.data(), jt.size() * sizeof(FdJob<T>), st)) ||
@@ -74,10 +104,11 @@ int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int steps, 
     const int ndom = nL > 0 ? nL : rem;
     const int stride = nL > 0 ? 7 : 37;
     const int nsamp = (ndom + stride - 1) / stride;
-    std::vector<hipEvent_t> ev(2 * nsamp);
+    std::vector<hipEvent_t> ev(2 * nsamp + 2);
     for (auto& e : ev) SWD_HIP(hipEventCreate(&e));
     rc = kOk;
     int launch = 0;
+    SWD_HIP(hipEventRecord(ev[2 * nsamp], st));
     for (int L = 0; L < nL && rc == kOk; ++L, ++launch) {
         const bool samp = L % stride == 0;
         if (samp) SWD_HIP(hipEventRecord(ev[2 * (L / stride)], st));
@@ -93,21 +124,28 @@ int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int steps, 
                  : launch_fwd_march<T>(st, djs.as<FdJob<T>>(), dbs.as<BlockMap>(), pss.nblocks, step, launch);
         if (samp && rc == kOk) SWD_HIP(hipEventRecord(ev[2 * (j / stride) + 1], st));
     }
+    if (rc == kOk) SWD_HIP(hipEventRecord(ev[2 * nsamp + 1], st));
     const hipError_t se = hipStreamSynchronize(st);
-    float tot = 0;
-    if (rc == kOk)
+    float tot = 0, span = 0;
+    if (rc == kOk) {
         for (int i = 0; i < nsamp; ++i) {
             float ms = 0;
             if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) == hipSuccess) tot += ms;
         }
+        if (hipEventElapsedTime(&span, ev[2 * nsamp], ev[2 * nsamp + 1]) != hipSuccess) span = 0;
+    }
     for (auto& e : ev) (void)hipEventDestroy(e);
     if (rc) return rc;
     if (se != hipSuccess) { set_error("FD march failed: %s", hipGetErrorString(se)); return kErrHip; }
-    if (kernel_us) *kernel_us = nsamp > 0 ? 1000.f * tot / nsamp : 0.f;
-    if (launches_tb) *launches_tb = nL > 0 ? NT : 1;
+    if (stats) {
+        stats->kernel_us = nsamp > 0 ? 1000.f * tot / nsamp : 0.f;
+        stats->span_ms = span;
+        stats->nt_block = nL > 0 ? NT : 1;
+        stats->rows = nL > 0 ? tb_rows : ss_rows;
+    }
     return kOk;
 }
-template int run_march<float>(hipStream_t, std::vector<FdJob<float>>&, bool, int, int, int, int, float*, int*);
-template int run_march<double>(hipStream_t, std::vector<FdJob<double>>&, bool, int, int, int, int, float*, int*);
+template int run_march<float>(hipStream_t, std::vector<FdJob<float>>&, bool, int, int, int, int, MarchStats*);
+template int run_march<double>(hipStream_t, std::vector<FdJob<double>>&, bool, int, int, int, int, MarchStats*);
 
 }  // namespace swd

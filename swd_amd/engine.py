@@ -122,15 +122,22 @@ class Engine:
     def set_tile_rows(self, tz: int):
         check(lib().swd_engine_set_tile_rows(self.h, int(tz)))
 
-    def set_time_block(self, nt_block: int, rows: int = 32):
-        """Steps per fused FD launch (1 = single-step kernels) and band rows."""
-        check(lib().swd_engine_set_time_block(self.h, int(nt_block), int(rows)))
+    def set_time_block(self, nt_block: int, rows: int = 0, phase: int = -1):
+        """Steps per fused FD launch (1 = single-step kernels) and band rows
+        (0 = chosen from occupancy) of the forward (phase 0), adjoint (1) or
+        both (-1) FD phases."""
+        check(lib().swd_engine_set_phase_time_block(self.h, int(phase), int(nt_block), int(rows)))
+
+    def autotune(self, steps: int = 60) -> dict:
+        """Time the FD march settings on this engine's shots and install the
+        fastest (see :func:`batch_autotune`)."""
+        return batch_autotune([self], steps)
 
     def timings(self) -> dict:
-        v = np.zeros(7)
-        check(lib().swd_engine_timings(self.h, ptr(v), 7))
+        v = np.zeros(10)
+        check(lib().swd_engine_timings(self.h, ptr(v), 10))
         return dict(zip(("fwd_ms", "disp_ms", "adj_ms", "asm_ms", "fwd_kernel_us", "adj_kernel_us",
-                         "steps_per_launch"), v))
+                         "fwd_steps_per_launch", "adj_steps_per_launch", "fwd_band_rows", "adj_band_rows"), v))
 
     def read(self, what: str) -> np.ndarray:
         q = self.problem
@@ -145,6 +152,23 @@ class Engine:
         if what == "gradient":
             return np.transpose(out, (0, 2, 1))     # -> [4][nz][nx]
         return out
+
+
+def batch_autotune(engines, steps: int = 60) -> dict:
+    """Autotune the FD march of a batch of engines: every time block 1..6 and
+    band heights splitting the tallest grid evenly are timed over `steps`
+    forward and adjoint steps on the engines' own buffers; the fastest
+    forward and adjoint settings are installed on every engine.  Overwrites
+    FD state, gathers, snapshots, adjoint sources and images."""
+    n = len(engines)
+    arr = (ctypes.c_void_p * n)(*[e.h.value for e in engines])
+    choice = np.zeros(4, dtype=np.int32)
+    table = np.zeros((256, 4))
+    nrows = np.zeros(1, dtype=np.int32)
+    check(lib().swd_batch_autotune(arr, n, int(steps), ptr(choice), ptr(table), table.shape[0], ptr(nrows)))
+    t = table[: min(int(nrows[0]), table.shape[0])]
+    return {"fwd": (int(choice[0]), int(choice[1])), "adj": (int(choice[2]), int(choice[3])),
+            "table": t, "candidates": int(nrows[0])}
 
 
 def batch_gradient(engines) -> tuple:

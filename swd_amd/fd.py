@@ -69,8 +69,9 @@ def eigen_e2drtm_double(iv, temp, ca, cl, cm, cm1, b, b1, adj, fux, fuz, bwx, bw
     return _e2drtm(np.float64, load().swd_e2drtm_f64, iv, temp, ca, cl, cm, cm1, b, b1, adj, fux, fuz, bwx, bwz)
 
 
-def set_time_block(nt_block: int, rows: int = 32) -> None:
-    """Steps per fused FD launch of the MEX entry points (1 = single-step kernels)."""
+def set_time_block(nt_block: int, rows: int = 0) -> None:
+    """Steps per fused FD launch of the MEX entry points (1 = single-step
+    kernels) and band rows of the time-blocked kernels (0 = automatic)."""
     check(load().swd_set_fd_time_block(int(nt_block), int(rows)))
 
 

@@ -28,10 +28,10 @@ def _fwd(args, nt_block, rows):
     try:
         return fd.staggerfd_eigen(*args)
     finally:
-        fd.set_time_block(4, 32)
+        fd.set_time_block(4, 0)
 
 
-@pytest.mark.parametrize("nt_block,rows", [(2, 32), (3, 9), (4, 32), (4, 7), (5, 26), (6, 40)])
+@pytest.mark.parametrize("nt_block,rows", [(2, 32), (3, 9), (4, 32), (4, 7), (4, 0), (5, 26), (6, 40)])
 def test_mex_forward_and_adjoint_bitwise(case, nt_block, rows):
     su1, sw1, S1 = _fwd(case, 1, 32)
     su, sw, S = _fwd(case, nt_block, rows)
@@ -48,7 +48,7 @@ def test_mex_forward_and_adjoint_bitwise(case, nt_block, rows):
     try:
         g = fd.e2drtm_eigen(*b)
     finally:
-        fd.set_time_block(4, 32)
+        fd.set_time_block(4, 0)
     for x, y in zip(g, g1):
         np.testing.assert_array_equal(x, y)
 
@@ -63,11 +63,33 @@ def test_engine_gradient_bitwise_across_time_blocks():
     e.set_time_block(1, 32)
     g1 = e.gradient()
     gath1 = e.read("gathers")
-    for nt_block, rows in ((4, 32), (5, 16), (3, 50)):
+    for nt_block, rows in ((4, 32), (5, 16), (3, 50), (6, 0)):
         e.set_time_block(nt_block, rows)
         g = e.gradient()
-        assert e.timings()["steps_per_launch"] == nt_block
+        t = e.timings()
+        assert t["fwd_steps_per_launch"] == nt_block and t["adj_steps_per_launch"] == nt_block
+        assert t["fwd_band_rows"] >= 8 if rows == 0 else t["fwd_band_rows"] == rows
         np.testing.assert_array_equal(e.read("gathers"), gath1)
         np.testing.assert_array_equal(g["cr_r"], g1["cr_r"])
         np.testing.assert_array_equal(g["dk_vs"], g1["dk_vs"])
+    # separate forward / adjoint settings
+    e.set_time_block(2, 20, phase=0)
+    e.set_time_block(5, 0, phase=1)
+    g = e.gradient()
+    t = e.timings()
+    assert (t["fwd_steps_per_launch"], t["fwd_band_rows"], t["adj_steps_per_launch"]) == (2, 20, 5)
+    np.testing.assert_array_equal(g["dk_vs"], g1["dk_vs"])
+    # the autotuner installs the fastest settings it timed; results unchanged
+    tune = e.autotune(steps=60)
+    tab = tune["table"]
+    assert tune["candidates"] == tab.shape[0] > 10
+    assert np.all(tab[:, 2] > 0) and np.all(tab[:, 3] > 0)
+    i_f, i_a = int(np.argmin(tab[:, 2])), int(np.argmin(tab[:, 3]))
+    assert tune["fwd"][0] == tab[i_f, 0] and tune["adj"][0] == tab[i_a, 0]
+    g = e.gradient()
+    t = e.timings()
+    assert t["fwd_steps_per_launch"] == tune["fwd"][0] and t["adj_steps_per_launch"] == tune["adj"][0]
+    np.testing.assert_array_equal(e.read("gathers"), gath1)
+    np.testing.assert_array_equal(g["cr_r"], g1["cr_r"])
+    np.testing.assert_array_equal(g["dk_vs"], g1["dk_vs"])
     e.close()
