@@ -81,6 +81,109 @@ def cpu_baseline(prob, mod, seconds_hint=15.0):
                       f"FP32 C restatement of the MEX statement loops, {wall:.1f} s on {platform.processor() or 'host'}"}
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def uneven_leg(steps=2):
+    """The uneven multi-model batch: 8 faulted layered models of different grid
+    size and nt (16 shots each) in shared FD launches, launch shape chosen by
+    the autotuner on the batch.  Also times the same 8 gradients one model
+    after another, checks each model's gradient alone == in the mix, and gives
+    the gather error of one shot per model against the FP64 MEX path."""
+    from swd_amd import fd, problems as P
+    from swd_amd.engine import Engine, batch_autotune, batch_gradient
+    models = P.uneven_models()
+    engs = []
+    for prob, mod in models:
+        e = Engine(prob)
+        e.set_wavelet(mod["s"])
+        e.set_model(mod["vp_true"], mod["vs_true"])
+        e.observe()
+        e.set_model(mod["vp"], mod["vs"])
+        engs.append(e)
+    progress("uneven batch: engines ready")
+    t0 = time.perf_counter()
+    tune = batch_autotune(engs)
+    tune_s = time.perf_counter() - t0
+    batch_gradient(engs)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        outs, _ = batch_gradient(engs)
+    t_batch = (time.perf_counter() - t0) / steps
+    cell_steps = 0
+    for e in engs:
+        i = e.info()
+        cell_steps += e.nshots * i["nzbc"] * i["nxbc"] * (2 * e.problem["nt"] - 1)
+    t0 = time.perf_counter()
+    alone = [e.gradient()["dk_vs"] for e in engs]
+    t_seq = time.perf_counter() - t0
+    same = all(np.array_equal(a, b, equal_nan=True) for a, b in zip(alone, outs))
+    errs = []
+    for (prob, mod), e in zip(models, engs):
+        gx = prob["gx0"] + prob["dg"] * np.arange(prob["ng"])
+        nt = prob["nt"]
+        _, sw, _ = fd.staggerfd_eigend(1, prob["nbc"], nt, prob["dt"] / prob["dx"], prob["dx"], prob["dt"],
+                                      int(prob["sx"][0]), 1, gx, np.ones(prob["ng"]), mod["s"], mod["vp"], mod["vs"],
+                                      1, 22, "w", 0, nt, snapshots=False)
+        g32 = e.read("gathers")[0].astype(np.float64)
+        errs.append(float(np.linalg.norm(g32 - sw) / np.linalg.norm(sw)))
+    for e in engs:
+        e.close()
+    return {"value": round(cell_steps / t_batch / 1e9, 3), "unit": "Gcell-steps/s",
+            "ms_per_batch_gradient": round(1000 * t_batch, 2),
+            "sequential_ms": round(1000 * t_seq, 2), "batch_speedup_vs_sequential": round(t_seq / t_batch, 3),
+            "models": [[p["nz"], p["nx"], p["nt"], len(p["sx"])] for p, _ in models],
+            "fd_launch": {"fwd": list(tune["fwd"]), "adj": list(tune["adj"]), "candidates": tune["candidates"],
+                          "tune_s": round(tune_s, 2)},
+            "alone_equals_in_mix": bool(same),
+            "gather_rel_l2_vs_fp64_max": max(errs),
+            "what": "8 models (nz x nx x nt x shots as listed), gradient of all 128 shots per step; error: FP32 "
+                    "batch gathers of shot 1 of each model vs the FP64 GPU MEX path"}
+
+
+def serving_leg(rates=(2.0, 10.0, 20.0), n=32, seed=0):
+    """The serving loop: Poisson arrivals of forward + gradient requests, each
+    a random model of the uneven set with 1..32 shots, through the
+    continuous-batching server (HBM-budget admission, up to 16 requests on
+    the GPU) and through the per-request path (the same server, one request
+    at a time) on the same arrival traces."""
+    from swd_amd import problems as P
+    from swd_amd.serve import Server, poisson_trace, request_templates, run_trace, subset
+    from swd_amd.engine import Engine
+    templates = request_templates(P.uneven_models(), shots=32)
+    legs = {}
+    check = None
+    for rate in rates:
+        trace = poisson_trace(templates, rate, n, seed)
+        row = {}
+        for name, slots in (("batched", 16), ("per_request", 1)):
+            srv = Server(max_active=slots, nt_block=(5, 4))
+            r = run_trace(srv, templates, trace)
+            st = srv.stats()
+            srv.close()
+            progress(f"serving {rate:g} req/s {name}: p50 {r['p50_ms']:.0f} ms p99 {r['p99_ms']:.0f} ms")
+            row[name] = {"p50_ms": round(r["p50_ms"], 1), "p99_ms": round(r["p99_ms"], 1),
+                         "mean_queued_ms": round(r["mean_queued_ms"], 1), "req_per_s": round(r["req_per_s"], 2),
+                         "gcell_steps_per_s": round(r["gcell_steps_per_s"], 2), "peak_active": st["peak_active"],
+                         "peak_hbm_gb": round(st["peak_bytes"] / 1e9, 1)}
+            if name == "batched" and check is None:
+                check = (trace[:3], r["results"][:3])
+        legs[f"{rate:g}"] = row
+    same = True
+    for (ta, ti, k), res in zip(*check):
+        q, mod, cr_r, cr_l = templates[ti]
+        e = Engine(subset(q, k))
+        e.set_wavelet(mod["s"])
+        e.set_observed_picks(cr_r[:, :k], cr_l[:, :k])
+        e.set_model(mod["vp"], mod["vs"])
+        same &= bool(np.array_equal(e.gradient()["dk_vs"], res["dk_vs"], equal_nan=True))
+        e.close()
+    return {"rates_req_per_s": legs, "requests_per_rate": n, "alone_equals_in_mix": same,
+            "what": "Poisson arrivals; request = forward + gradient of one of the 8 uneven models with 1..32 shots; "
+                    "latency = submit to result on the host; per_request = one request on the GPU at a time"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +197,8 @@ def main():
     ap.add_argument("--band-rows", type=int, default=0, help="band rows of the time-blocked kernels (0: automatic)")
     ap.add_argument("--plain-shots", type=int, default=2, help="shots for the unfused comparator (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven multi-model batch leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the shot-parallel multi-GPU "
@@ -163,7 +268,7 @@ def main():
                                   "nt=6000, dt=8e-5, 600 receivers; Radon 151 freq x 701 vel; beta=10",
                       "shots": S, "nt": nt, "grid": [info["nzbc"], info["nxbc"]], "parallelism": "1 GPU, shot batch"},
            "shots_grad_per_s": round(S / (wall / args.steps), 3),
-           "latency_ms": {"p50": round(1000 * float(np.percentile(per_step, 50)), 2),
+           "gradient_latency_ms": {"p50": round(1000 * float(np.percentile(per_step, 50)), 2),
                           "p99": round(1000 * float(np.percentile(per_step, 99)), 2)},
            "stage_ms": {k: round(float(np.mean([s[k] for s in stage])), 2)
                         for k in ("fwd_ms", "disp_ms", "adj_ms", "asm_ms")},
@@ -188,9 +293,18 @@ def main():
                              "what": f"unfused per-statement kernels, one shot per launch, {args.plain_shots} shots",
                              "speedup": round(value / plain, 2)}
         e2.close()
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(prob, mod)
     eng.close()
+    progress(f"headline: {value:.1f} Gcell-steps/s")
+    if not args.no_uneven:
+        out["uneven_batch"] = uneven_leg()
+    if not args.no_serving:
+        out["serving"] = serving_leg()
+        mid = out["serving"]["rates_req_per_s"].get("10", {}).get("batched")
+        if mid:
+            out["req_latency_ms"] = {"rate_req_per_s": 10, "p50": mid["p50_ms"], "p99": mid["p99_ms"]}
+    if not args.no_cpu_baseline:
+        progress("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(prob, mod)
     print(json.dumps(out), flush=True)
 
 

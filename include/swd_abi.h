@@ -230,6 +230,44 @@ int swd_engine_read(swd_engine *e, int what, void *dst, long long bytes);
  * march-kernel duration. */
 int swd_batch_gradient(swd_engine **engines, int n, double **dk_vs, double *kernel_us);
 
+/* Number of Radon frequency bins (rows of the pick arrays) of a problem. */
+int swd_problem_npair(const swd_problem *p, int *npair);
+
+/*
+ * Continuous-batching gradient server (no reference counterpart: the serving
+ * form of the per-iteration gradient of SWD.m:171-201 for callers that solve
+ * many models).  A scheduler thread admits queued requests in arrival order
+ * while a slot is free and the request's buffers fit in the device budget;
+ * admitted requests' shots advance together in shared FD launches, joining
+ * and leaving the launch sequence independently, and each request's result
+ * is bitwise the result of swd_engine_gradient on the same inputs.
+ *
+ * swd_server_create: hbm_bytes = device budget for request buffers (0: free
+ * memory less 2 GiB); max_active = requests on the GPU at once (1: one
+ * request at a time); nt_block_fwd / nt_block_adj = FD steps per launch
+ * (2..6).  swd_server_destroy fails requests still queued, waits for the
+ * admitted ones and frees everything.
+ * swd_server_submit: copies the problem (and its arrays), the current model
+ * vp / vs (nz x nx column-major), the wavelet (nt) and the observed picks
+ * cr0_r / cr0_l (npair x nshots column-major); returns a ticket.
+ * swd_server_wait: waits up to timeout_ms (< 0: no limit) for the ticket
+ * (SWD_ERR_BUSY when it is still running), copies dk_vs (nz x nx column-major)
+ * and the pick misfits res_r / res_l (nshots) (NULL skips), and times_ms[4] =
+ * {queued, on the GPU, total, submit time since server start}, then forgets
+ * the ticket; returns the request's own status (SWD_ERR_PICK etc.).
+ * swd_server_stats: vals[0..9] = queued, active, completed, failed, device
+ * bytes in use, device budget, FD launches, peak active, peak device bytes,
+ * launch-table rebuilds.
+ */
+typedef struct swd_server swd_server;
+int swd_server_create(long long hbm_bytes, int max_active, int nt_block_fwd, int nt_block_adj, swd_server **out);
+int swd_server_destroy(swd_server *s);
+int swd_server_submit(swd_server *s, const swd_problem *p, const double *vp, const double *vs,
+                      const double *wavelet, const double *cr0_r, const double *cr0_l, long long *ticket);
+int swd_server_wait(swd_server *s, long long ticket, double timeout_ms, double *dk_vs, double *res_r, double *res_l,
+                    double *times_ms);
+int swd_server_stats(swd_server *s, long long *vals, int n);
+
 #ifdef __cplusplus
 }
 #endif

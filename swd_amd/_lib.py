@@ -67,6 +67,12 @@ SIGNATURES = {
     "swd_engine_timings": (_I, [_P, _P, _I]),
     "swd_engine_read": (_I, [_P, _I, _P, ctypes.c_longlong]),
     "swd_batch_gradient": (_I, [_P, _I, _P, _P]),
+    "swd_problem_npair": (_I, [_P, _P]),
+    "swd_server_create": (_I, [ctypes.c_longlong, _I, _I, _I, _P]),
+    "swd_server_destroy": (_I, [_P]),
+    "swd_server_submit": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "swd_server_wait": (_I, [_P, ctypes.c_longlong, ctypes.c_double, _P, _P, _P, _P]),
+    "swd_server_stats": (_I, [_P, _P, _I]),
 }
 
 _lock = threading.Lock()

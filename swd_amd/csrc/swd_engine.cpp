@@ -24,6 +24,14 @@ Engine::~Engine()
     if (own_stream && st) (void)hipStreamDestroy(st);
 }
 
+int problem_geometry(const swd_problem& p, DispGeom& g)
+{
+    memset(&g, 0, sizeof g);
+    g.nt = p.nt; g.ng = p.ng; g.nshots = p.nshots; g.df = p.df; g.dt = p.dt; g.fmin = p.fmin; g.fmax = p.fmax;
+    g.vmin = p.vmin; g.vmax = p.vmax; g.ds = p.ds; g.offset = p.offset; g.m = p.m; g.M = p.M;
+    return disp_geometry(g);
+}
+
 int Engine::create(const swd_problem* prob, hipStream_t stream)
 {
     if (!prob) { set_error("null problem"); return kErrArg; }
@@ -69,10 +77,7 @@ int Engine::create(const swd_problem* prob, hipStream_t stream)
     tz = fd_tile_rows();
     // dispersion plan
     DispGeom g;
-    memset(&g, 0, sizeof g);
-    g.nt = p.nt; g.ng = p.ng; g.nshots = p.nshots; g.df = p.df; g.dt = p.dt; g.fmin = p.fmin; g.fmax = p.fmax;
-    g.vmin = p.vmin; g.vmax = p.vmax; g.ds = p.ds; g.offset = p.offset; g.m = p.m; g.M = p.M;
-    int rc = disp_geometry(g);
+    int rc = problem_geometry(p, g);
     if (rc) return rc;
     if (p.M == 0) { set_error("M must be non-zero"); return kErrArg; }
     npair = g.npair;
@@ -368,6 +373,16 @@ int swd_engine_create(const swd_problem* p, swd_engine** out)
     const int rc = h->e.create(p, nullptr);
     if (rc) { delete h; return rc; }
     *out = h;
+    return kOk;
+}
+
+int swd_problem_npair(const swd_problem* p, int* npair)
+{
+    if (!p || !npair) { set_error("null argument"); return kErrArg; }
+    DispGeom g;
+    const int rc = problem_geometry(*p, g);
+    if (rc) return rc;
+    *npair = g.npair;
     return kOk;
 }
 

@@ -12,6 +12,9 @@
 
 namespace swd {
 
+// Radon geometry (frequency bins, velocities) of a problem.
+int problem_geometry(const swd_problem& p, DispGeom& g);
+
 int launch_assemble_gradient(hipStream_t st, const double* imgs, int nshots, int nz, int nx, const double* vs,
                              double* dk, double* gcl, double* gcm, double* gil);
 

@@ -27,30 +27,87 @@ int prepare_model(const double* vp, const double* vs, int nz, int nx, int nbc, d
                   double* temp, double* ca, double* cl, double* cm, double* cm1, double* b, double* b1);
 int ricker(double f, double dt, int nt, double* w);
 
+// First-fit sub-allocator over one allocation made once (device memory via
+// hipMalloc or pinned host memory via hipHostMalloc), with coalescing frees.
+// The gradient server builds every request's buffers from one, so admitting
+// and retiring requests never calls hipMalloc or hipFree.  Thread-safe.
+class Arena {
+public:
+    Arena() = default;
+    Arena(const Arena&) = delete;
+    Arena& operator=(const Arena&) = delete;
+    ~Arena();
+    int init_device(size_t bytes);
+    int init_host(size_t bytes);
+    void* take(size_t n);             // nullptr when no free block fits
+    void give(void* p, size_t n);
+    size_t capacity() const { return cap_; }
+    size_t used() const;
+    size_t largest_free() const;
+
+private:
+    struct Impl;
+    Impl* impl_ = nullptr;
+    char* base_ = nullptr;
+    size_t cap_ = 0;
+    bool host_ = false;
+};
+
+// DevBuf allocations made by this thread while an ArenaScope is alive come
+// from its arena instead of hipMalloc.
+Arena* current_arena();
+struct ArenaScope {
+    Arena* prev;
+    explicit ArenaScope(Arena* a);
+    ~ArenaScope();
+};
+
 // Device allocation owned by the host object (freed on destruction).
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    Arena* arena = nullptr;  // set when the memory came from an arena
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), arena(o.arena)
+    {
+        o.p = nullptr; o.bytes = 0; o.arena = nullptr;
+    }
     DevBuf& operator=(DevBuf&& o) noexcept
     {
-        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        if (this != &o) {
+            release();
+            p = o.p; bytes = o.bytes; arena = o.arena;
+            o.p = nullptr; o.bytes = 0; o.arena = nullptr;
+        }
         return *this;
     }
     ~DevBuf() { release(); }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (arena) arena->give(p, bytes);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         bytes = 0;
+        arena = nullptr;
     }
     int alloc(size_t n)
     {
         release();
         if (n == 0) n = 256;
+        if (Arena* a = current_arena()) {
+            p = a->take(n);
+            if (!p) {
+                set_error("arena has no free block of %zu bytes (%zu of %zu in use)", n, a->used(), a->capacity());
+                return kErrNoMem;
+            }
+            arena = a;
+            bytes = n;
+            return kOk;
+        }
         if (hipMalloc(&p, n) != hipSuccess) {
             (void)hipGetLastError();
             p = nullptr;

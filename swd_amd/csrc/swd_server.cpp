@@ -1,0 +1,759 @@
+// Continuous-batching gradient server.
+//
+// Requests are forward + gradient solves (the per-iteration gradient of
+// SWD.m:171-201) of any model size, nt and shot count.  A scheduler thread
+// owns one FD stream on which every admitted request's shots advance in
+// shared time-blocked launches: a request joins the running launch sequence
+// at the next launch after admission (its jobs carry the global step and
+// launch index at which they joined) and leaves it when its phase ends, so
+// requests of different sizes and arrival times overlap on the device
+// without waiting for each other.  The per-request stages between the FD
+// phases (Radon, picking, adjoint sources; image reduction and read-back)
+// run on side streams, ordered after the request's last FD launch by events.
+//
+// Memory: every request's buffers come from one device arena (the HBM
+// budget), results from one pinned host arena.  A request is admitted, in
+// arrival order, when a slot is free and its buffers fit; it leaves all of
+// its memory resident on the device until it finishes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "swd_engine.h"
+
+namespace swd {
+
+// ---------------------------------------------------------------------------
+// Arena
+// ---------------------------------------------------------------------------
+struct Arena::Impl {
+    mutable std::mutex m;
+    std::map<size_t, size_t> free_;  // offset -> size
+    size_t used = 0;
+};
+
+static constexpr size_t kArenaAlign = 256;
+static size_t arena_round(size_t n) { return (n + kArenaAlign - 1) / kArenaAlign * kArenaAlign; }
+
+Arena::~Arena()
+{
+    if (base_) {
+        if (host_) (void)hipHostFree(base_);
+        else (void)hipFree(base_);
+    }
+    delete impl_;
+}
+
+int Arena::init_device(size_t bytes)
+{
+    bytes = arena_round(std::max<size_t>(bytes, kArenaAlign));
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("device arena of %zu bytes: hipMalloc failed", bytes);
+        return kErrNoMem;
+    }
+    base_ = (char*)p;
+    cap_ = bytes;
+    host_ = false;
+    impl_ = new Impl;
+    impl_->free_[0] = bytes;
+    return kOk;
+}
+
+int Arena::init_host(size_t bytes)
+{
+    bytes = arena_round(std::max<size_t>(bytes, kArenaAlign));
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("pinned host arena of %zu bytes: hipHostMalloc failed", bytes);
+        return kErrNoMem;
+    }
+    base_ = (char*)p;
+    cap_ = bytes;
+    host_ = true;
+    impl_ = new Impl;
+    impl_->free_[0] = bytes;
+    return kOk;
+}
+
+void* Arena::take(size_t n)
+{
+    if (!impl_) return nullptr;
+    n = arena_round(std::max<size_t>(n, 1));
+    std::lock_guard<std::mutex> lk(impl_->m);
+    for (auto it = impl_->free_.begin(); it != impl_->free_.end(); ++it)
+        if (it->second >= n) {
+            const size_t off = it->first, sz = it->second;
+            impl_->free_.erase(it);
+            if (sz > n) impl_->free_[off + n] = sz - n;
+            impl_->used += n;
+            return base_ + off;
+        }
+    return nullptr;
+}
+
+void Arena::give(void* p, size_t n)
+{
+    if (!impl_ || !p) return;
+    n = arena_round(std::max<size_t>(n, 1));
+    size_t off = (size_t)((char*)p - base_);
+    std::lock_guard<std::mutex> lk(impl_->m);
+    impl_->used -= n;
+    auto nx = impl_->free_.lower_bound(off);
+    if (nx != impl_->free_.end() && off + n == nx->first) {  // merge with the following block
+        n += nx->second;
+        nx = impl_->free_.erase(nx);
+    }
+    if (nx != impl_->free_.begin()) {  // merge with the preceding block
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second == off) {
+            pv->second += n;
+            return;
+        }
+    }
+    impl_->free_[off] = n;
+}
+
+size_t Arena::used() const
+{
+    if (!impl_) return 0;
+    std::lock_guard<std::mutex> lk(impl_->m);
+    return impl_->used;
+}
+
+size_t Arena::largest_free() const
+{
+    if (!impl_) return 0;
+    std::lock_guard<std::mutex> lk(impl_->m);
+    size_t b = 0;
+    for (const auto& f : impl_->free_) b = std::max(b, f.second);
+    return b;
+}
+
+static thread_local Arena* t_arena = nullptr;
+Arena* current_arena() { return t_arena; }
+ArenaScope::ArenaScope(Arena* a) : prev(t_arena) { t_arena = a; }
+ArenaScope::~ArenaScope() { t_arena = prev; }
+
+// ---------------------------------------------------------------------------
+// Server
+// ---------------------------------------------------------------------------
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+enum Phase { kQueued = 0, kFwd, kDisp, kAdj, kFinish, kDone, kFailed };
+
+struct Request {
+    long long id = 0;
+    swd_problem p{};
+    std::vector<int> sx, sz, isx;
+    std::vector<double> vp, vs, wav, cr0r, cr0l;
+    std::unique_ptr<Engine> eng;
+    std::atomic<int> phase{kQueued};  // read by waiting callers
+    int start = 0, launch0 = 0, launches = 0;  // current FD phase: join step / launch, launches it needs
+    std::vector<FdJob<float>> jobs;            // current FD phase's jobs
+    hipEvent_t ev_fd = nullptr, ev = nullptr;  // last FD launch of a phase; end of a side-stream stage
+    hipStream_t side = nullptr;
+    double* hout = nullptr;  // pinned: dk_vs (nz*nx), res (2*nshots), status (2*nshots ints, after)
+    size_t hbytes = 0;
+    int rc = kOk;
+    std::string err;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    std::vector<double> dk, res_r, res_l;
+};
+
+constexpr int kMaxJobs = 1 << 14;       // shots in one FD phase across requests
+constexpr int kMaxBlocks = 1 << 20;     // workgroups of one launch
+constexpr int kStage = 4;               // pinned staging slots per FD phase
+constexpr int kMaxLaunchesPerEpoch = 32;
+
+}  // namespace
+
+struct Server {
+    Arena dev, host;
+    hipStream_t fd = nullptr;
+    std::vector<hipStream_t> side;
+    int next_side = 0;
+    int nt_blk[2] = {4, 4};
+    int max_active = 16;
+    Clock::time_point t0 = Clock::now();
+
+    std::mutex m;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> queue;
+    std::vector<Request*> active;
+    std::map<long long, std::unique_ptr<Request>> reqs;
+    long long next_id = 1;
+    bool stop = false;
+    std::thread th;
+    std::string fatal;
+
+    // FD launch sequence of each phase (0 forward, 1 adjoint)
+    int gstep[2] = {0, 0}, glaunch[2] = {0, 0};
+    std::vector<Request*> members[2];
+    bool dirty[2] = {false, false};
+    DevBuf djobs[2], dbmap[2];
+    int nblocks[2] = {0, 0};
+    char* stage[2][kStage] = {};
+    hipEvent_t stage_ev[2][kStage] = {};
+    bool stage_used[2][kStage] = {};
+    int stage_next[2] = {0, 0};
+    hipEvent_t epoch_ev[2] = {};
+    bool epoch_used[2] = {false, false};
+    int epoch = 0;
+
+    // statistics
+    long long n_done = 0, n_failed = 0, n_launch = 0, peak_active = 0, n_rebuild = 0;
+    size_t peak_bytes = 0;
+
+    double now_ms() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
+
+    int init(size_t hbm, int maxa, int ntf, int nta);
+    void shutdown();
+    void loop();
+    void fail(Request* r, int rc, const char* what);
+    void admit();
+    void progress();
+    int rebuild(int ph);
+    int end_phase(Request* r, int ph);
+    void retire(Request* r);
+};
+
+int Server::init(size_t hbm, int maxa, int ntf, int nta)
+{
+    max_active = std::max(1, maxa);
+    nt_blk[0] = ntf;
+    nt_blk[1] = nta;
+    size_t fr = 0, tot = 0;
+    SWD_HIP(hipMemGetInfo(&fr, &tot));
+    const size_t reserve = (size_t)2 << 30;
+    const size_t avail = fr > reserve + ((size_t)64 << 20) ? fr - reserve : fr / 2;
+    if (hbm == 0 || hbm > avail) hbm = avail;
+    int rc = dev.init_device(hbm);
+    // one allocation of (nearly) all free memory can be refused: step down
+    while (rc == kErrNoMem && hbm > ((size_t)2 << 30)) {
+        hbm = hbm / 8 * 7;
+        rc = dev.init_device(hbm);
+    }
+    if (rc || (rc = host.init_host((size_t)256 << 20))) return rc;
+    SWD_HIP(hipStreamCreateWithFlags(&fd, hipStreamNonBlocking));
+    side.resize(4);
+    for (auto& s : side) SWD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    for (int ph = 0; ph < 2; ++ph) {
+        if ((rc = djobs[ph].alloc(kMaxJobs * sizeof(FdJob<float>))) ||
+            (rc = dbmap[ph].alloc(kMaxBlocks * sizeof(BlockMap))))
+            return rc;
+        for (int k = 0; k < kStage; ++k) {
+            void* h = nullptr;
+            SWD_HIP(hipHostMalloc(&h, kMaxJobs * sizeof(FdJob<float>) + kMaxBlocks * sizeof(BlockMap),
+                                  hipHostMallocDefault));
+            stage[ph][k] = (char*)h;
+            SWD_HIP(hipEventCreateWithFlags(&stage_ev[ph][k], hipEventDisableTiming));
+        }
+    }
+    for (auto& e : epoch_ev) SWD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    th = std::thread([this] { loop(); });
+    return kOk;
+}
+
+void Server::shutdown()
+{
+    {
+        std::lock_guard<std::mutex> lk(m);
+        stop = true;
+        for (Request* r : queue) {
+            r->phase = kFailed;
+            r->rc = kErrArg;
+            r->err = "server shut down before the request was admitted";
+        }
+        queue.clear();
+    }
+    cv_work.notify_all();
+    if (th.joinable()) th.join();
+    cv_done.notify_all();
+    if (fd) (void)hipStreamSynchronize(fd);
+    for (auto s : side) (void)hipStreamSynchronize(s);
+    for (auto& kv : reqs) {
+        Request* r = kv.second.get();
+        if (r->eng) { r->eng->st = nullptr; r->eng.reset(); }
+        if (r->ev) (void)hipEventDestroy(r->ev);
+        if (r->ev_fd) (void)hipEventDestroy(r->ev_fd);
+        r->ev = r->ev_fd = nullptr;
+    }
+    for (int ph = 0; ph < 2; ++ph)
+        for (int k = 0; k < kStage; ++k) {
+            if (stage[ph][k]) (void)hipHostFree(stage[ph][k]);
+            if (stage_ev[ph][k]) (void)hipEventDestroy(stage_ev[ph][k]);
+        }
+    for (auto& e : epoch_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto s : side) (void)hipStreamDestroy(s);
+    if (fd) (void)hipStreamDestroy(fd);
+}
+
+void Server::fail(Request* r, int rc, const char* what)
+{
+    r->rc = rc;
+    r->err = std::string(what) + ": " + last_error();
+    if (r->eng) {
+        if (r->side) (void)hipStreamSynchronize(r->side);
+        r->eng->st = nullptr;
+        r->eng.reset();
+    }
+    if (r->hout) { host.give(r->hout, r->hbytes); r->hout = nullptr; }
+    r->t_done = now_ms();
+    std::lock_guard<std::mutex> lk(m);
+    r->phase = kFailed;
+    active.erase(std::remove(active.begin(), active.end(), r), active.end());
+    ++n_failed;
+    cv_done.notify_all();
+}
+
+// Admit queued requests in arrival order while a slot is free and the head
+// request's buffers fit in the device arena.
+void Server::admit()
+{
+    for (;;) {
+        Request* r = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(m);
+            if (queue.empty() || (int)active.size() >= max_active) return;
+            r = queue.front();
+        }
+        r->p.sx = r->sx.data();
+        r->p.sz = r->sz.data();
+        r->p.is_index = r->isx.data();
+        r->side = side[next_side];
+        std::unique_ptr<Engine> e(new Engine);
+        int rc;
+        {
+            ArenaScope scope(&dev);
+            rc = e->create(&r->p, r->side);
+        }
+        if (rc == kErrNoMem) {
+            bool alone;
+            {
+                std::lock_guard<std::mutex> lk(m);
+                alone = active.empty();
+            }
+            e->st = nullptr;
+            e.reset();
+            if (!alone) return;  // wait for memory to free up
+            {
+                std::lock_guard<std::mutex> lk(m);
+                queue.pop_front();
+            }
+            fail(r, rc, "request does not fit in the server's HBM budget");
+            continue;
+        }
+        {
+            std::lock_guard<std::mutex> lk(m);
+            queue.pop_front();
+            active.push_back(r);
+            peak_active = std::max<long long>(peak_active, (long long)active.size());
+        }
+        next_side = (next_side + 1) % (int)side.size();
+        r->eng = std::move(e);
+        if (rc) { fail(r, rc, "engine create"); continue; }
+        Engine& g = *r->eng;
+        const size_t S = r->p.nshots;
+        r->hbytes = g.splane * sizeof(double) + 2 * S * sizeof(double) + 2 * S * sizeof(int);
+        r->hout = (double*)host.take(r->hbytes);
+        if (!r->hout) { set_error("pinned result arena exhausted"); fail(r, kErrNoMem, "admit"); continue; }
+        if ((rc = g.set_model(r->vp.data(), r->vs.data())) || (rc = g.set_wavelet(r->wav.data())) ||
+            (rc = g.set_observed_picks(r->cr0r.data(), r->cr0l.data()))) {
+            fail(r, rc, "admit");
+            continue;
+        }
+        if (hipEventCreateWithFlags(&r->ev_fd, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess) {
+            set_error("hipEventCreate failed");
+            fail(r, kErrHip, "admit");
+            continue;
+        }
+        peak_bytes = std::max(peak_bytes, dev.used());
+        r->t_admit = now_ms();
+        // join the forward launch sequence at its next launch
+        r->jobs.clear();
+        g.make_jobs(r->jobs, false, true);
+        r->phase = kFwd;
+        members[0].push_back(r);
+        dirty[0] = true;
+    }
+}
+
+// Move requests whose side-stream stage has finished to their next phase.
+void Server::progress()
+{
+    std::vector<Request*> act;
+    {
+        std::lock_guard<std::mutex> lk(m);
+        act = active;
+    }
+    for (Request* r : act) {
+        if (r->phase != kDisp && r->phase != kFinish) continue;
+        const hipError_t q = hipEventQuery(r->ev);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) {
+            set_error("stage event: %s", hipGetErrorString(q));
+            fail(r, kErrHip, "side stream");
+            continue;
+        }
+        if (r->phase == kDisp) {
+            r->jobs.clear();
+            r->eng->make_jobs(r->jobs, true, true);
+            r->phase = kAdj;
+            members[1].push_back(r);
+            dirty[1] = true;
+        } else {
+            retire(r);
+        }
+    }
+}
+
+void Server::retire(Request* r)
+{
+    Engine& g = *r->eng;
+    const size_t S = r->p.nshots;
+    const double* res = r->hout + g.splane;
+    const int* stat = (const int*)(res + 2 * S);
+    r->dk.assign(r->hout, r->hout + g.splane);
+    r->res_r.assign(res, res + S);
+    r->res_l.assign(res + S, res + 2 * S);
+    r->rc = kOk;
+    for (size_t c = 0; c < 2 * S; ++c)
+        if (g.disp.cols[c].enabled && stat[c]) {
+            r->rc = kErrPick;
+            char b[160];
+            snprintf(b, sizeof b, "pick of shot %zu (%s side) kept fewer than two columns", c % S,
+                     c < S ? "right" : "left");
+            r->err = b;
+            break;
+        }
+    host.give(r->hout, r->hbytes);
+    r->hout = nullptr;
+    g.st = nullptr;  // its side-stream work has finished (event above)
+    r->eng.reset();  // buffers return to the arena
+    (void)hipEventDestroy(r->ev);
+    (void)hipEventDestroy(r->ev_fd);
+    r->ev = r->ev_fd = nullptr;
+    r->t_done = now_ms();
+    std::lock_guard<std::mutex> lk(m);
+    r->phase = r->rc == kOk ? kDone : kFailed;
+    if (r->rc == kOk) ++n_done;
+    else ++n_failed;
+    active.erase(std::remove(active.begin(), active.end(), r), active.end());
+    cv_done.notify_all();
+}
+
+// Rebuild the job table of one FD phase after requests joined or left: new
+// members take the current global step and launch index as their origin,
+// their wavefield state (and gathers or images) is cleared in stream order
+// before the next launch, and the table goes up through a pinned staging
+// slot in stream order, so launches already queued are unaffected.
+int Server::rebuild(int ph)
+{
+    dirty[ph] = false;
+    ++n_rebuild;
+    if (members[ph].empty()) gstep[ph] = glaunch[ph] = 0;
+    if (gstep[ph] > (1 << 29)) {
+        // keep the step and launch counters small: the kernels use only
+        // differences (step - start, launch - launch0)
+        int s0 = gstep[ph], l0 = glaunch[ph];
+        for (Request* r : members[ph])
+            if (r->launches > 0) { s0 = std::min(s0, r->start); l0 = std::min(l0, r->launch0); }
+        l0 &= ~1;
+        gstep[ph] -= s0;
+        glaunch[ph] -= l0;
+        for (Request* r : members[ph])
+            if (r->launches > 0) {
+                r->start -= s0;
+                r->launch0 -= l0;
+                for (auto& J : r->jobs) { J.start = r->start; J.launch0 = r->launch0; }
+            }
+    }
+    std::vector<FdJob<float>> jobs;
+    for (Request* r : members[ph]) {
+        if (r->launches == 0) {  // new member
+            r->start = gstep[ph];
+            r->launch0 = glaunch[ph];
+            const int steps = ph == 0 ? r->p.nt : r->p.nt - 1;
+            r->launches = div_up(steps, nt_blk[ph]);
+            Engine& g = *r->eng;
+            SWD_HIP(hipMemsetAsync(g.state.p, 0, g.state.bytes, fd));
+            if (ph == 0) SWD_HIP(hipMemsetAsync(g.traces.p, 0, g.traces.bytes, fd));
+            else SWD_HIP(hipMemsetAsync(g.imgs.p, 0, g.imgs.bytes, fd));
+            for (auto& J : r->jobs) {
+                J.start = r->start;
+                J.launch0 = r->launch0;
+            }
+        }
+        jobs.insert(jobs.end(), r->jobs.begin(), r->jobs.end());
+    }
+    nblocks[ph] = 0;
+    if (jobs.empty()) return kOk;
+    if ((int)jobs.size() > kMaxJobs) { set_error("more than %d shots in one FD phase", kMaxJobs); return kErrArg; }
+    const int NT = nt_blk[ph];
+    const int rows = auto_band_rows(jobs, ph == 1, NT);
+    MarchPlan plan;
+    int rc = build_march_plan(jobs, rows, plan, kMarchLanes - 2 * NT);
+    if (rc) return rc;
+    if (plan.nblocks > kMaxBlocks) { set_error("FD launch of %d workgroups exceeds %d", plan.nblocks, kMaxBlocks); return kErrArg; }
+    const int k = stage_next[ph];
+    stage_next[ph] = (k + 1) % kStage;
+    if (stage_used[ph][k]) SWD_HIP(hipEventSynchronize(stage_ev[ph][k]));
+    char* h = stage[ph][k];
+    const size_t jb = jobs.size() * sizeof(FdJob<float>), bb = plan.bmap.size() * sizeof(BlockMap);
+    memcpy(h, jobs.data(), jb);
+    memcpy(h + kMaxJobs * sizeof(FdJob<float>), plan.bmap.data(), bb);
+    SWD_HIP(hipMemcpyAsync(djobs[ph].p, h, jb, hipMemcpyHostToDevice, fd));
+    SWD_HIP(hipMemcpyAsync(dbmap[ph].p, h + kMaxJobs * sizeof(FdJob<float>), bb, hipMemcpyHostToDevice, fd));
+    SWD_HIP(hipEventRecord(stage_ev[ph][k], fd));
+    stage_used[ph][k] = true;
+    nblocks[ph] = plan.nblocks;
+    return kOk;
+}
+
+// A request's FD phase has been fully launched: order its next stage on its
+// side stream after the last launch.
+int Server::end_phase(Request* r, int ph)
+{
+    Engine& g = *r->eng;
+    SWD_HIP(hipEventRecord(r->ev_fd, fd));
+    SWD_HIP(hipStreamWaitEvent(r->side, r->ev_fd, 0));
+    int rc;
+    r->launches = 0;
+    if (ph == 0) {
+        if ((rc = g.dispersion()) || (rc = g.adjoint_source())) return rc;
+        SWD_HIP(hipEventRecord(r->ev, r->side));
+        r->phase = kDisp;
+    } else {
+        if ((rc = g.assemble())) return rc;
+        const size_t S = r->p.nshots;
+        double* res = r->hout + g.splane;
+        SWD_HIP(hipMemcpyAsync(r->hout, g.outs.p, g.splane * sizeof(double), hipMemcpyDeviceToHost, r->side));
+        SWD_HIP(hipMemcpyAsync(res, g.disp.res.p, 2 * S * sizeof(double), hipMemcpyDeviceToHost, r->side));
+        SWD_HIP(hipMemcpyAsync(res + 2 * S, g.disp.status.p, 2 * S * sizeof(int), hipMemcpyDeviceToHost, r->side));
+        SWD_HIP(hipEventRecord(r->ev, r->side));
+        r->phase = kFinish;
+    }
+    return kOk;
+}
+
+void Server::loop()
+{
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(m);
+            if (active.empty() && queue.empty()) {
+                if (stop) break;
+                cv_work.wait_for(lk, std::chrono::milliseconds(20));
+                continue;
+            }
+        }
+        progress();
+        admit();
+        int rc = kOk;
+        for (int ph = 0; ph < 2 && rc == kOk; ++ph)
+            if (dirty[ph] || gstep[ph] > (1 << 29)) rc = rebuild(ph);
+        if (rc == kOk && members[0].empty() && members[1].empty()) {
+            std::this_thread::sleep_for(std::chrono::microseconds(100));  // only side-stream stages pending
+            continue;
+        }
+        // launches until the nearest phase end, at most kMaxLaunchesPerEpoch
+        int E = kMaxLaunchesPerEpoch;
+        for (int ph = 0; ph < 2; ++ph)
+            for (Request* r : members[ph]) E = std::min(E, r->launch0 + r->launches - glaunch[ph]);
+        E = std::max(E, 1);
+        for (int k = 0; k < E && rc == kOk; ++k)
+            for (int ph = 0; ph < 2 && rc == kOk; ++ph) {
+                if (members[ph].empty()) continue;
+                rc = ph == 0 ? launch_fwd_tb<float>(fd, djobs[0].as<FdJob<float>>(), dbmap[0].as<BlockMap>(),
+                                                    nblocks[0], gstep[0], glaunch[0], nt_blk[0])
+                             : launch_adj_tb<float>(fd, djobs[1].as<FdJob<float>>(), dbmap[1].as<BlockMap>(),
+                                                    nblocks[1], gstep[1], glaunch[1], nt_blk[1]);
+                gstep[ph] += nt_blk[ph];
+                ++glaunch[ph];
+                ++n_launch;
+            }
+        if (rc != kOk) {  // a launch failed: every request on the FD stream fails with it
+            std::vector<Request*> act;
+            {
+                std::lock_guard<std::mutex> lk(m);
+                act = active;
+            }
+            (void)hipStreamSynchronize(fd);
+            for (Request* r : act) fail(r, rc, "FD launch");
+            members[0].clear();
+            members[1].clear();
+            continue;
+        }
+        for (int ph = 0; ph < 2; ++ph) {
+            std::vector<Request*> keep;
+            for (Request* r : members[ph]) {
+                if (glaunch[ph] - r->launch0 < r->launches) { keep.push_back(r); continue; }
+                const int erc = end_phase(r, ph);
+                if (erc) fail(r, erc, ph == 0 ? "dispersion stage" : "assembly stage");
+                dirty[ph] = true;
+            }
+            members[ph].swap(keep);
+        }
+        // keep at most two epochs of launches queued ahead of the device
+        hipError_t he = hipEventRecord(epoch_ev[epoch], fd);
+        epoch_used[epoch] = true;
+        epoch ^= 1;
+        if (he == hipSuccess && epoch_used[epoch]) he = hipEventSynchronize(epoch_ev[epoch]);
+        if (he != hipSuccess) {
+            set_error("FD stream: %s", hipGetErrorString(he));
+            std::vector<Request*> act;
+            {
+                std::lock_guard<std::mutex> lk(m);
+                act = active;
+            }
+            for (Request* r : act) fail(r, kErrHip, "FD stream");
+            members[0].clear();
+            members[1].clear();
+        }
+    }
+}
+
+}  // namespace swd
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+using namespace swd;
+
+struct swd_server {
+    Server s;
+};
+
+extern "C" {
+
+int swd_server_create(long long hbm_bytes, int max_active, int nt_block_fwd, int nt_block_adj, swd_server** out)
+{
+    if (!out) { set_error("null output handle"); return kErrArg; }
+    *out = nullptr;
+    if (hbm_bytes < 0 || max_active < 1 || nt_block_fwd < 2 || nt_block_fwd > 6 || nt_block_adj < 2 ||
+        nt_block_adj > 6) {
+        set_error("server needs hbm_bytes >= 0, max_active >= 1 and time blocks of 2..6 steps");
+        return kErrArg;
+    }
+    swd_server* h = new (std::nothrow) swd_server;
+    if (!h) { set_error("out of host memory"); return kErrNoMem; }
+    const int rc = h->s.init((size_t)hbm_bytes, max_active, nt_block_fwd, nt_block_adj);
+    if (rc) {
+        h->s.shutdown();
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return kOk;
+}
+
+int swd_server_destroy(swd_server* h)
+{
+    if (!h) return kErrArg;
+    h->s.shutdown();
+    delete h;
+    return kOk;
+}
+
+int swd_server_submit(swd_server* h, const swd_problem* p, const double* vp, const double* vs, const double* wavelet,
+                      const double* cr0_r, const double* cr0_l, long long* ticket)
+{
+    if (!h || !p || !vp || !vs || !wavelet || !cr0_r || !cr0_l || !ticket) { set_error("null argument"); return kErrArg; }
+    if (p->nshots < 1 || p->nz < 1 || p->nx < 1 || p->nt < 2 || !p->sx || !p->sz) {
+        set_error("bad problem");
+        return kErrArg;
+    }
+    int npair = 0;
+    int rc = swd_problem_npair(p, &npair);
+    if (rc) return rc;
+    std::unique_ptr<Request> r(new (std::nothrow) Request);
+    if (!r) { set_error("out of host memory"); return kErrNoMem; }
+    r->p = *p;
+    const size_t S = p->nshots, n = (size_t)p->nz * p->nx, np = (size_t)npair * S;
+    r->sx.assign(p->sx, p->sx + S);
+    r->sz.assign(p->sz, p->sz + S);
+    if (p->is_index) r->isx.assign(p->is_index, p->is_index + S);
+    else for (size_t s = 0; s < S; ++s) r->isx.push_back((int)s + 1);
+    r->vp.assign(vp, vp + n);
+    r->vs.assign(vs, vs + n);
+    r->wav.assign(wavelet, wavelet + p->nt);
+    r->cr0r.assign(cr0_r, cr0_r + np);
+    r->cr0l.assign(cr0_l, cr0_l + np);
+    Server& s = h->s;
+    {
+        std::lock_guard<std::mutex> lk(s.m);
+        if (s.stop) { set_error("server is shutting down"); return kErrArg; }
+        r->id = s.next_id++;
+        r->t_submit = s.now_ms();
+        *ticket = r->id;
+        s.queue.push_back(r.get());
+        s.reqs[r->id] = std::move(r);
+    }
+    s.cv_work.notify_all();
+    return kOk;
+}
+
+int swd_server_wait(swd_server* h, long long ticket, double timeout_ms, double* dk_vs, double* res_r, double* res_l,
+                    double* times_ms)
+{
+    if (!h) return kErrArg;
+    Server& s = h->s;
+    std::unique_lock<std::mutex> lk(s.m);
+    auto it = s.reqs.find(ticket);
+    if (it == s.reqs.end()) { set_error("unknown ticket %lld", ticket); return kErrArg; }
+    Request* r = it->second.get();
+    auto finished = [&] { return r->phase == kDone || r->phase == kFailed; };
+    if (timeout_ms < 0) s.cv_done.wait(lk, finished);
+    else if (!s.cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), finished)) {
+        set_error("request %lld still running", ticket);
+        return kErrBusy;
+    }
+    if (times_ms) {
+        times_ms[0] = r->t_admit > 0 ? r->t_admit - r->t_submit : r->t_done - r->t_submit;
+        times_ms[1] = r->t_admit > 0 ? r->t_done - r->t_admit : 0;
+        times_ms[2] = r->t_done - r->t_submit;
+        times_ms[3] = r->t_submit;
+    }
+    const int rc = r->rc;
+    if (rc == kOk) {
+        if (dk_vs) std::copy(r->dk.begin(), r->dk.end(), dk_vs);
+        if (res_r) std::copy(r->res_r.begin(), r->res_r.end(), res_r);
+        if (res_l) std::copy(r->res_l.begin(), r->res_l.end(), res_l);
+    } else {
+        set_error("%s", r->err.c_str());
+    }
+    s.reqs.erase(it);
+    return rc;
+}
+
+int swd_server_stats(swd_server* h, long long* vals, int n)
+{
+    if (!h || !vals) return kErrArg;
+    Server& s = h->s;
+    std::lock_guard<std::mutex> lk(s.m);
+    const long long v[10] = {(long long)s.queue.size(), (long long)s.active.size(), s.n_done, s.n_failed,
+                             (long long)s.dev.used(), (long long)s.dev.capacity(), s.n_launch, s.peak_active,
+                             (long long)s.peak_bytes, s.n_rebuild};
+    for (int i = 0; i < n && i < 10; ++i) vals[i] = v[i];
+    return kOk;
+}
+
+}  // extern "C"

@@ -29,22 +29,36 @@ def lib():
     return load()
 
 
+def make_problem(q: dict):
+    """The swd_problem struct of a problem dict, with the int32 arrays it
+    points to (keep them alive as long as the struct is used)."""
+    sx = np.ascontiguousarray(q["sx"], dtype=np.int32)
+    sz = np.ascontiguousarray(q["sz"], dtype=np.int32)
+    isx = None if q.get("is_index") is None else np.ascontiguousarray(q["is_index"], dtype=np.int32)
+    p = Problem()
+    for k in PROBLEM_KEYS:
+        setattr(p, k, q.get(k, 0) if k != "fd_order" else q.get(k, 22))
+    p.nshots = int(sx.size)
+    p.sx = sx.ctypes.data
+    p.sz = sz.ctypes.data
+    p.is_index = None if isx is None else isx.ctypes.data
+    return p, (sx, sz, isx)
+
+
+def problem_npair(q: dict) -> int:
+    """Radon frequency bins of a problem (rows of its pick arrays)."""
+    p, keep = make_problem(q)
+    n = ctypes.c_int(0)
+    check(lib().swd_problem_npair(ctypes.byref(p), ctypes.byref(n)))
+    return int(n.value)
+
+
 class Engine:
     """Batched SWD engine for one model (FP32 FD + Radon, FP64 reductions)."""
 
     def __init__(self, problem: dict):
         self.problem = dict(problem)
-        q = self.problem
-        self._sx = np.ascontiguousarray(q["sx"], dtype=np.int32)
-        self._sz = np.ascontiguousarray(q["sz"], dtype=np.int32)
-        self._is = None if q.get("is_index") is None else np.ascontiguousarray(q["is_index"], dtype=np.int32)
-        p = Problem()
-        for k in PROBLEM_KEYS:
-            setattr(p, k, q.get(k, 0) if k != "fd_order" else q.get(k, 22))
-        p.nshots = int(self._sx.size)
-        p.sx = self._sx.ctypes.data
-        p.sz = self._sz.ctypes.data
-        p.is_index = None if self._is is None else self._is.ctypes.data
+        p, (self._sx, self._sz, self._is) = make_problem(self.problem)
         self._p = p
         h = ctypes.c_void_p()
         check(lib().swd_engine_create(ctypes.byref(p), ctypes.byref(h)))

@@ -65,9 +65,10 @@ def faulted_layers(nz, nx, rng, vmin=150.0, vmax=700.0, throw_m=10.0, dx=0.5):
     return vs
 
 
-def uneven_models(seed=20261203, shots=16, sizes=(200, 300, 450, 600, 800, 900, 1000, 1200), nt_range=(2000, 8000)):
+def uneven_models(seed=20261203, shots=16, sizes=(200, 300, 450, 600, 800, 900, 1000, 1200), nt_range=(2000, 8000),
+                  f0=20.0):
     """Eight models of different size and nt, each with `shots` shots (dx 0.5 m,
-    30-cell borders, dt 1e-4 s, 20 Hz Ricker), faulted layered Vs 150-700 m/s."""
+    30-cell borders, dt 1e-4 s, f0 Hz Ricker), faulted layered Vs 150-700 m/s."""
     rng = np.random.default_rng(seed)
     out = []
     for nx in sizes:
@@ -84,5 +85,5 @@ def uneven_models(seed=20261203, shots=16, sizes=(200, 300, 450, 600, 800, 900, 
                     vmin=100.0, vmax=800.0, ds=dx, offset=nx, m=0, M=1, ini=10, w=0.0, beta=10.0, ns=nx,
                     is_index=sx)
         out.append((prob, {"vs_true": vs, "vp_true": 1.73 * vs, "vs": vs_cur, "vp": 1.73 * vs_cur,
-                           "s": ricker(20.0, dt, nt)}))
+                           "s": ricker(f0, dt, nt)}))
     return out
