@@ -144,44 +144,44 @@ def uneven_leg(steps=2):
 
 def serving_leg(rates=(2.0, 10.0, 20.0), n=32, seed=0):
     """The serving loop: Poisson arrivals of forward + gradient requests, each
-    a random model of the uneven set with 1..32 shots, through the
-    continuous-batching server (HBM-budget admission, up to 16 requests on
-    the GPU) and through the per-request path (the same server, one request
-    at a time) on the same arrival traces."""
+    a random model of the uneven set with 1..32 shots.  "batched" is the
+    continuous-batching server (HBM-budget admission, up to 16 requests on the
+    GPU, least remaining work first); "one_slot" the same server with one
+    request on the GPU at a time; "per_request" the plain path, an Engine per
+    request one after another.  Same arrival trace for all three."""
     from swd_amd import problems as P
-    from swd_amd.serve import Server, poisson_trace, request_templates, run_trace, subset
-    from swd_amd.engine import Engine
+    from swd_amd.serve import Server, poisson_trace, request_templates, run_trace, run_trace_per_request
     templates = request_templates(P.uneven_models(), shots=32)
     legs = {}
-    check = None
+    same = True
     for rate in rates:
         trace = poisson_trace(templates, rate, n, seed)
         row = {}
-        for name, slots in (("batched", 16), ("per_request", 1)):
-            srv = Server(max_active=slots, nt_block=(5, 4))
-            r = run_trace(srv, templates, trace)
-            st = srv.stats()
-            srv.close()
+        outs = {}
+        for name, slots in (("batched", 16), ("one_slot", 1), ("per_request", 0)):
+            if slots:
+                srv = Server(max_active=slots, nt_block=(5, 4))
+                r = run_trace(srv, templates, trace)
+                st = srv.stats()
+                srv.close()
+            else:
+                r = run_trace_per_request(templates, trace)
+                st = {"peak_active": 1, "peak_bytes": 0}
+            outs[name] = r["results"]
             progress(f"serving {rate:g} req/s {name}: p50 {r['p50_ms']:.0f} ms p99 {r['p99_ms']:.0f} ms")
             row[name] = {"p50_ms": round(r["p50_ms"], 1), "p99_ms": round(r["p99_ms"], 1),
                          "mean_queued_ms": round(r["mean_queued_ms"], 1), "req_per_s": round(r["req_per_s"], 2),
-                         "gcell_steps_per_s": round(r["gcell_steps_per_s"], 2), "peak_active": st["peak_active"],
-                         "peak_hbm_gb": round(st["peak_bytes"] / 1e9, 1)}
-            if name == "batched" and check is None:
-                check = (trace[:3], r["results"][:3])
+                         "gcell_steps_per_s": round(r["gcell_steps_per_s"], 2), "peak_active": st["peak_active"]}
+            if slots:
+                row[name]["peak_hbm_gb"] = round(st["peak_bytes"] / 1e9, 1)
+        # every request: the server's result is the plain path's result, bitwise
+        for a, b in zip(outs["batched"], outs["per_request"]):
+            same &= bool(np.array_equal(a["dk_vs"], b["dk_vs"], equal_nan=True))
         legs[f"{rate:g}"] = row
-    same = True
-    for (ta, ti, k), res in zip(*check):
-        q, mod, cr_r, cr_l = templates[ti]
-        e = Engine(subset(q, k))
-        e.set_wavelet(mod["s"])
-        e.set_observed_picks(cr_r[:, :k], cr_l[:, :k])
-        e.set_model(mod["vp"], mod["vs"])
-        same &= bool(np.array_equal(e.gradient()["dk_vs"], res["dk_vs"], equal_nan=True))
-        e.close()
     return {"rates_req_per_s": legs, "requests_per_rate": n, "alone_equals_in_mix": same,
             "what": "Poisson arrivals; request = forward + gradient of one of the 8 uneven models with 1..32 shots; "
-                    "latency = submit to result on the host; per_request = one request on the GPU at a time"}
+                    "latency = arrival to result on the host; alone_equals_in_mix: every batched result equals the "
+                    "per_request result bitwise"}
 
 
 def main():

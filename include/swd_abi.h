@@ -240,7 +240,9 @@ int swd_problem_npair(const swd_problem *p, int *npair);
  * while a slot is free and the request's buffers fit in the device budget;
  * admitted requests' shots advance together in shared FD launches, joining
  * and leaving the launch sequence independently, and each request's result
- * is bitwise the result of swd_engine_gradient on the same inputs.
+ * is bitwise the result of swd_engine_gradient on the same inputs.  Each
+ * launch advances the requests with the least remaining work until their
+ * shots fill the GPU once; the others wait with their wavefields resident.
  *
  * swd_server_create: hbm_bytes = device budget for request buffers (0: free
  * memory less 2 GiB); max_active = requests on the GPU at once (1: one
@@ -267,6 +269,11 @@ int swd_server_submit(swd_server *s, const swd_problem *p, const double *vp, con
 int swd_server_wait(swd_server *s, long long ticket, double timeout_ms, double *dk_vs, double *res_r, double *res_l,
                     double *times_ms);
 int swd_server_stats(swd_server *s, long long *vals, int n);
+/* Launch selection: requests are advanced least remaining work first until
+ * their shots fill this many resident waves (0: the device's own count, the
+ * default); the others wait with their wavefields resident, at most 150 ms
+ * per FD phase. */
+int swd_server_set_wave_slots(swd_server *s, long long waves);
 
 #ifdef __cplusplus
 }

@@ -164,60 +164,74 @@ struct Request {
     std::vector<double> vp, vs, wav, cr0r, cr0l;
     std::unique_ptr<Engine> eng;
     std::atomic<int> phase{kQueued};  // read by waiting callers
-    int start = 0, launch0 = 0, launches = 0;  // current FD phase: join step / launch, launches it needs
-    std::vector<FdJob<float>> jobs;            // current FD phase's jobs
+    // current FD phase: launches it needs and has been given, whether its jobs
+    // are in the phase's launch table, waves per launch, cell-steps per launch
+    int total = 0, done = 0;
+    bool sel = false;
+    long long waves = 0;
+    double work_per_launch = 0, work_after = 0;
+    double paused_since = -1;
+    bool pinned = false;  // waited kMaxPauseMs: stays selected until its phase ends
+    std::vector<FdJob<float>> jobs;
     hipEvent_t ev_fd = nullptr, ev = nullptr;  // last FD launch of a phase; end of a side-stream stage
     hipStream_t side = nullptr;
-    double* hout = nullptr;  // pinned: dk_vs (nz*nx), res (2*nshots), status (2*nshots ints, after)
+    double* hout = nullptr;  // pinned: dk_vs (nz*nx), res (2*nshots), status (2*nshots ints)
     size_t hbytes = 0;
     int rc = kOk;
     std::string err;
     double t_submit = 0, t_admit = 0, t_done = 0;
     std::vector<double> dk, res_r, res_l;
+    double remaining() const { return (total - done) * work_per_launch + work_after; }
 };
 
-constexpr int kMaxJobs = 1 << 14;       // shots in one FD phase across requests
+constexpr int kMaxJobs = 1 << 14;       // shots in one FD launch table
 constexpr int kMaxBlocks = 1 << 20;     // workgroups of one launch
 constexpr int kStage = 4;               // pinned staging slots per FD phase
 constexpr int kMaxLaunchesPerEpoch = 32;
+constexpr double kMaxPauseMs = 150.0;   // a request paused this long is selected until its phase ends
 
 }  // namespace
 
 struct Server {
     Arena dev, host;
-    hipStream_t fd = nullptr;
+    hipStream_t fd[2] = {nullptr, nullptr};  // forward and adjoint launch sequences
     std::vector<hipStream_t> side;
     int next_side = 0;
     int nt_blk[2] = {4, 4};
+    long long slots = 0;  // resident waves of the GPU for the time-blocked kernels
+    std::atomic<long long> slots_override{0};  // set by the caller (0: slots)
     int max_active = 16;
     Clock::time_point t0 = Clock::now();
 
     std::mutex m;
-    std::condition_variable cv_work, cv_done;
-    std::deque<Request*> queue;
-    std::vector<Request*> active;
+    std::condition_variable cv_work, cv_done, cv_admit;
+    std::deque<Request*> queue;   // submitted, not yet admitted
+    std::vector<Request*> ready;  // admitted by the admission thread, not yet seen by the scheduler
+    std::vector<Request*> active; // admitted, not finished (admission slots)
     std::map<long long, std::unique_ptr<Request>> reqs;
     long long next_id = 1;
     bool stop = false;
-    std::thread th;
-    std::string fatal;
+    bool mem_blocked = false;  // the head of the queue waits for memory
+    std::thread th, th_admit;
 
-    // FD launch sequence of each phase (0 forward, 1 adjoint)
+    // scheduler-thread state
     int gstep[2] = {0, 0}, glaunch[2] = {0, 0};
     std::vector<Request*> members[2];
-    bool dirty[2] = {false, false};
+    bool reselect = false;
+    std::vector<Request*> table_of[2];  // members in the current launch table
     DevBuf djobs[2], dbmap[2];
     int nblocks[2] = {0, 0};
     char* stage[2][kStage] = {};
     hipEvent_t stage_ev[2][kStage] = {};
     bool stage_used[2][kStage] = {};
     int stage_next[2] = {0, 0};
-    hipEvent_t epoch_ev[2] = {};
-    bool epoch_used[2] = {false, false};
-    int epoch = 0;
+    hipEvent_t epoch_ev[2][2] = {};
+    bool epoch_used[2][2] = {};
+    int epoch[2] = {0, 0};
 
     // statistics
-    long long n_done = 0, n_failed = 0, n_launch = 0, peak_active = 0, n_rebuild = 0;
+    long long n_done = 0, n_failed = 0, peak_active = 0;
+    std::atomic<long long> n_launch{0}, n_rebuild{0};
     size_t peak_bytes = 0;
 
     double now_ms() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
@@ -225,12 +239,14 @@ struct Server {
     int init(size_t hbm, int maxa, int ntf, int nta);
     void shutdown();
     void loop();
+    void admission_loop();
     void fail(Request* r, int rc, const char* what);
-    void admit();
     void progress();
+    void select(double now);
     int rebuild(int ph);
     int end_phase(Request* r, int ph);
     void retire(Request* r);
+    void fail_all(int rc, const char* what);
 };
 
 int Server::init(size_t hbm, int maxa, int ntf, int nta)
@@ -250,7 +266,12 @@ int Server::init(size_t hbm, int maxa, int ntf, int nta)
         rc = dev.init_device(hbm);
     }
     if (rc || (rc = host.init_host((size_t)256 << 20))) return rc;
-    SWD_HIP(hipStreamCreateWithFlags(&fd, hipStreamNonBlocking));
+    int dev_id = 0, cus = 0;
+    SWD_HIP(hipGetDevice(&dev_id));
+    SWD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id));
+    const int nb = std::min(tb_blocks_per_cu<float>(false, ntf), tb_blocks_per_cu<float>(true, nta));
+    slots = (long long)std::max(cus, 1) * std::max(nb, 1) * kMarchWaves;
+    for (auto& s : fd) SWD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     side.resize(4);
     for (auto& s : side) SWD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     for (int ph = 0; ph < 2; ++ph) {
@@ -264,9 +285,10 @@ int Server::init(size_t hbm, int maxa, int ntf, int nta)
             stage[ph][k] = (char*)h;
             SWD_HIP(hipEventCreateWithFlags(&stage_ev[ph][k], hipEventDisableTiming));
         }
+        for (auto& e : epoch_ev[ph]) SWD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
-    for (auto& e : epoch_ev) SWD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     th = std::thread([this] { loop(); });
+    th_admit = std::thread([this] { admission_loop(); });
     return kOk;
 }
 
@@ -276,16 +298,19 @@ void Server::shutdown()
         std::lock_guard<std::mutex> lk(m);
         stop = true;
         for (Request* r : queue) {
-            r->phase = kFailed;
             r->rc = kErrArg;
             r->err = "server shut down before the request was admitted";
+            r->phase = kFailed;
         }
         queue.clear();
     }
+    cv_admit.notify_all();
     cv_work.notify_all();
+    if (th_admit.joinable()) th_admit.join();
     if (th.joinable()) th.join();
     cv_done.notify_all();
-    if (fd) (void)hipStreamSynchronize(fd);
+    for (auto s : fd)
+        if (s) (void)hipStreamSynchronize(s);
     for (auto s : side) (void)hipStreamSynchronize(s);
     for (auto& kv : reqs) {
         Request* r = kv.second.get();
@@ -294,15 +319,17 @@ void Server::shutdown()
         if (r->ev_fd) (void)hipEventDestroy(r->ev_fd);
         r->ev = r->ev_fd = nullptr;
     }
-    for (int ph = 0; ph < 2; ++ph)
+    for (int ph = 0; ph < 2; ++ph) {
         for (int k = 0; k < kStage; ++k) {
             if (stage[ph][k]) (void)hipHostFree(stage[ph][k]);
             if (stage_ev[ph][k]) (void)hipEventDestroy(stage_ev[ph][k]);
         }
-    for (auto& e : epoch_ev)
-        if (e) (void)hipEventDestroy(e);
+        for (auto& e : epoch_ev[ph])
+            if (e) (void)hipEventDestroy(e);
+    }
     for (auto s : side) (void)hipStreamDestroy(s);
-    if (fd) (void)hipStreamDestroy(fd);
+    for (auto s : fd)
+        if (s) (void)hipStreamDestroy(s);
 }
 
 void Server::fail(Request* r, int rc, const char* what)
@@ -316,22 +343,31 @@ void Server::fail(Request* r, int rc, const char* what)
     }
     if (r->hout) { host.give(r->hout, r->hbytes); r->hout = nullptr; }
     r->t_done = now_ms();
-    std::lock_guard<std::mutex> lk(m);
-    r->phase = kFailed;
-    active.erase(std::remove(active.begin(), active.end(), r), active.end());
-    ++n_failed;
+    {
+        std::lock_guard<std::mutex> lk(m);
+        active.erase(std::remove(active.begin(), active.end(), r), active.end());
+        ++n_failed;
+        mem_blocked = false;
+        r->phase = kFailed;
+    }
     cv_done.notify_all();
+    cv_admit.notify_all();
 }
 
-// Admit queued requests in arrival order while a slot is free and the head
-// request's buffers fit in the device arena.
-void Server::admit()
+// Admission thread: in arrival order, while a slot is free, build the head
+// request's engine in the device arena and upload its model, wavelet and
+// observed picks; a request whose buffers do not fit waits until a running
+// request returns memory (or fails when nothing else holds any).
+void Server::admission_loop()
 {
     for (;;) {
         Request* r = nullptr;
         {
-            std::lock_guard<std::mutex> lk(m);
-            if (queue.empty() || (int)active.size() >= max_active) return;
+            std::unique_lock<std::mutex> lk(m);
+            cv_admit.wait(lk, [&] {
+                return stop || (!queue.empty() && (int)active.size() < max_active && !mem_blocked);
+            });
+            if (stop) return;
             r = queue.front();
         }
         r->p.sx = r->sx.data();
@@ -345,18 +381,15 @@ void Server::admit()
             rc = e->create(&r->p, r->side);
         }
         if (rc == kErrNoMem) {
-            bool alone;
-            {
-                std::lock_guard<std::mutex> lk(m);
-                alone = active.empty();
-            }
             e->st = nullptr;
             e.reset();
-            if (!alone) return;  // wait for memory to free up
-            {
-                std::lock_guard<std::mutex> lk(m);
-                queue.pop_front();
+            std::unique_lock<std::mutex> lk(m);
+            if (!active.empty()) {
+                mem_blocked = true;  // cleared when a request returns its memory
+                continue;
             }
+            queue.pop_front();
+            lk.unlock();
             fail(r, rc, "request does not fit in the server's HBM budget");
             continue;
         }
@@ -381,19 +414,37 @@ void Server::admit()
         }
         if (hipEventCreateWithFlags(&r->ev_fd, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
             set_error("hipEventCreate failed");
             fail(r, kErrHip, "admit");
             continue;
         }
-        peak_bytes = std::max(peak_bytes, dev.used());
-        r->t_admit = now_ms();
-        // join the forward launch sequence at its next launch
         r->jobs.clear();
         g.make_jobs(r->jobs, false, true);
-        r->phase = kFwd;
-        members[0].push_back(r);
-        dirty[0] = true;
+        {
+            std::lock_guard<std::mutex> lk(m);
+            peak_bytes = std::max(peak_bytes, dev.used());
+            r->t_admit = now_ms();
+            ready.push_back(r);
+        }
+        cv_work.notify_all();
     }
+}
+
+// Start a request's FD phase: its launch count, its per-launch work and its
+// wave count for the selection.
+static void begin_phase(Request* r, int ph, int NT)
+{
+    Engine& g = *r->eng;
+    const int steps = ph == 0 ? r->p.nt : r->p.nt - 1;
+    r->total = div_up(steps, NT);
+    r->done = 0;
+    r->sel = false;
+    r->pinned = false;
+    const double cells = (double)r->p.nshots * g.nzbc * g.nxbc;
+    r->work_per_launch = cells * NT;
+    r->work_after = ph == 0 ? cells * (r->p.nt - 1) : 0.0;
+    r->waves = (long long)r->p.nshots * div_up(g.nxbc - 2, kMarchLanes - 2 * NT) * div_up(g.nzbc - 2, 32);
 }
 
 // Move requests whose side-stream stage has finished to their next phase.
@@ -403,6 +454,14 @@ void Server::progress()
     {
         std::lock_guard<std::mutex> lk(m);
         act = active;
+        for (Request* r : ready) {
+            r->phase = kFwd;
+            begin_phase(r, 0, nt_blk[0]);
+            r->paused_since = now_ms();
+            members[0].push_back(r);
+            reselect = true;
+        }
+        ready.clear();
     }
     for (Request* r : act) {
         if (r->phase != kDisp && r->phase != kFinish) continue;
@@ -417,8 +476,10 @@ void Server::progress()
             r->jobs.clear();
             r->eng->make_jobs(r->jobs, true, true);
             r->phase = kAdj;
+            begin_phase(r, 1, nt_blk[1]);
+            r->paused_since = now_ms();
             members[1].push_back(r);
-            dirty[1] = true;
+            reselect = true;
         } else {
             retire(r);
         }
@@ -452,62 +513,85 @@ void Server::retire(Request* r)
     (void)hipEventDestroy(r->ev_fd);
     r->ev = r->ev_fd = nullptr;
     r->t_done = now_ms();
-    std::lock_guard<std::mutex> lk(m);
-    r->phase = r->rc == kOk ? kDone : kFailed;
-    if (r->rc == kOk) ++n_done;
-    else ++n_failed;
-    active.erase(std::remove(active.begin(), active.end(), r), active.end());
+    {
+        std::lock_guard<std::mutex> lk(m);
+        if (r->rc == kOk) ++n_done;
+        else ++n_failed;
+        active.erase(std::remove(active.begin(), active.end(), r), active.end());
+        mem_blocked = false;
+        r->phase = r->rc == kOk ? kDone : kFailed;
+    }
     cv_done.notify_all();
+    cv_admit.notify_all();
 }
 
-// Rebuild the job table of one FD phase after requests joined or left: new
-// members take the current global step and launch index as their origin,
-// their wavefield state (and gathers or images) is cleared in stream order
-// before the next launch, and the table goes up through a pinned staging
-// slot in stream order, so launches already queued are unaffected.
+// Choose the requests that the next launches advance: least remaining work
+// first until the selected shots fill the GPU's resident waves once (then
+// every further request would only lengthen the launches), plus every
+// request that has waited kMaxPauseMs in its current phase (it then stays
+// selected until the phase ends, which bounds the wait of large requests).
+void Server::select(double now)
+{
+    reselect = false;
+    std::vector<std::pair<int, Request*>> c;
+    for (int ph = 0; ph < 2; ++ph)
+        for (Request* r : members[ph]) c.push_back({ph, r});
+    std::stable_sort(c.begin(), c.end(), [](const std::pair<int, Request*>& a, const std::pair<int, Request*>& b) {
+        return a.second->remaining() < b.second->remaining();
+    });
+    long long waves = 0;
+    const long long cap = slots_override.load() > 0 ? slots_override.load() : slots;
+    for (auto& pr : c) {
+        Request* r = pr.second;
+        if (r->paused_since >= 0 && now - r->paused_since > kMaxPauseMs) r->pinned = true;
+        const bool s = waves < cap || r->pinned;
+        if (s) waves += r->waves;
+        if (s != r->sel) {
+            r->sel = s;
+            r->paused_since = s ? -1.0 : now;
+        }
+    }
+    for (int ph = 0; ph < 2; ++ph) {
+        std::vector<Request*> t;
+        for (Request* r : members[ph])
+            if (r->sel) t.push_back(r);
+        if (t != table_of[ph]) {
+            table_of[ph] = t;
+            const int rc = rebuild(ph);
+            if (rc) fail_all(rc, "FD launch table");
+        }
+    }
+}
+
+// Upload the launch table of one FD phase.  Each selected job's origin is
+// set from the launches it has been given, so that the kernels see its own
+// step (step - start = done*NT) and ping-pong parity (launch - launch0 =
+// done) however long it was paused; a job's state is cleared before its
+// first launch.  The table goes up through a pinned staging slot in stream
+// order, so launches already queued keep the table they were queued with.
 int Server::rebuild(int ph)
 {
-    dirty[ph] = false;
     ++n_rebuild;
     if (members[ph].empty()) gstep[ph] = glaunch[ph] = 0;
-    if (gstep[ph] > (1 << 29)) {
-        // keep the step and launch counters small: the kernels use only
-        // differences (step - start, launch - launch0)
-        int s0 = gstep[ph], l0 = glaunch[ph];
-        for (Request* r : members[ph])
-            if (r->launches > 0) { s0 = std::min(s0, r->start); l0 = std::min(l0, r->launch0); }
-        l0 &= ~1;
-        gstep[ph] -= s0;
-        glaunch[ph] -= l0;
-        for (Request* r : members[ph])
-            if (r->launches > 0) {
-                r->start -= s0;
-                r->launch0 -= l0;
-                for (auto& J : r->jobs) { J.start = r->start; J.launch0 = r->launch0; }
-            }
-    }
+    if (gstep[ph] > (1 << 29)) { gstep[ph] = 0; glaunch[ph] = 0; }  // only differences matter
+    const int NT = nt_blk[ph];
     std::vector<FdJob<float>> jobs;
-    for (Request* r : members[ph]) {
-        if (r->launches == 0) {  // new member
-            r->start = gstep[ph];
-            r->launch0 = glaunch[ph];
-            const int steps = ph == 0 ? r->p.nt : r->p.nt - 1;
-            r->launches = div_up(steps, nt_blk[ph]);
+    for (Request* r : table_of[ph]) {
+        if (r->done == 0) {
             Engine& g = *r->eng;
-            SWD_HIP(hipMemsetAsync(g.state.p, 0, g.state.bytes, fd));
-            if (ph == 0) SWD_HIP(hipMemsetAsync(g.traces.p, 0, g.traces.bytes, fd));
-            else SWD_HIP(hipMemsetAsync(g.imgs.p, 0, g.imgs.bytes, fd));
-            for (auto& J : r->jobs) {
-                J.start = r->start;
-                J.launch0 = r->launch0;
-            }
+            SWD_HIP(hipMemsetAsync(g.state.p, 0, g.state.bytes, fd[ph]));
+            if (ph == 0) SWD_HIP(hipMemsetAsync(g.traces.p, 0, g.traces.bytes, fd[ph]));
+            else SWD_HIP(hipMemsetAsync(g.imgs.p, 0, g.imgs.bytes, fd[ph]));
+        }
+        for (auto& J : r->jobs) {
+            J.start = gstep[ph] - r->done * NT;
+            J.launch0 = glaunch[ph] - r->done;
         }
         jobs.insert(jobs.end(), r->jobs.begin(), r->jobs.end());
     }
     nblocks[ph] = 0;
     if (jobs.empty()) return kOk;
-    if ((int)jobs.size() > kMaxJobs) { set_error("more than %d shots in one FD phase", kMaxJobs); return kErrArg; }
-    const int NT = nt_blk[ph];
+    if ((int)jobs.size() > kMaxJobs) { set_error("more than %d shots in one FD launch", kMaxJobs); return kErrArg; }
     const int rows = auto_band_rows(jobs, ph == 1, NT);
     MarchPlan plan;
     int rc = build_march_plan(jobs, rows, plan, kMarchLanes - 2 * NT);
@@ -520,9 +604,9 @@ int Server::rebuild(int ph)
     const size_t jb = jobs.size() * sizeof(FdJob<float>), bb = plan.bmap.size() * sizeof(BlockMap);
     memcpy(h, jobs.data(), jb);
     memcpy(h + kMaxJobs * sizeof(FdJob<float>), plan.bmap.data(), bb);
-    SWD_HIP(hipMemcpyAsync(djobs[ph].p, h, jb, hipMemcpyHostToDevice, fd));
-    SWD_HIP(hipMemcpyAsync(dbmap[ph].p, h + kMaxJobs * sizeof(FdJob<float>), bb, hipMemcpyHostToDevice, fd));
-    SWD_HIP(hipEventRecord(stage_ev[ph][k], fd));
+    SWD_HIP(hipMemcpyAsync(djobs[ph].p, h, jb, hipMemcpyHostToDevice, fd[ph]));
+    SWD_HIP(hipMemcpyAsync(dbmap[ph].p, h + kMaxJobs * sizeof(FdJob<float>), bb, hipMemcpyHostToDevice, fd[ph]));
+    SWD_HIP(hipEventRecord(stage_ev[ph][k], fd[ph]));
     stage_used[ph][k] = true;
     nblocks[ph] = plan.nblocks;
     return kOk;
@@ -533,10 +617,10 @@ int Server::rebuild(int ph)
 int Server::end_phase(Request* r, int ph)
 {
     Engine& g = *r->eng;
-    SWD_HIP(hipEventRecord(r->ev_fd, fd));
+    SWD_HIP(hipEventRecord(r->ev_fd, fd[ph]));
     SWD_HIP(hipStreamWaitEvent(r->side, r->ev_fd, 0));
     int rc;
-    r->launches = 0;
+    r->sel = false;
     if (ph == 0) {
         if ((rc = g.dispersion()) || (rc = g.adjoint_source())) return rc;
         SWD_HIP(hipEventRecord(r->ev, r->side));
@@ -554,79 +638,89 @@ int Server::end_phase(Request* r, int ph)
     return kOk;
 }
 
+void Server::fail_all(int rc, const char* what)
+{
+    for (auto s : fd) (void)hipStreamSynchronize(s);
+    std::vector<Request*> act;
+    {
+        std::lock_guard<std::mutex> lk(m);
+        act = active;
+    }
+    for (Request* r : act) fail(r, rc, what);
+    for (int ph = 0; ph < 2; ++ph) {
+        members[ph].clear();
+        table_of[ph].clear();
+        nblocks[ph] = 0;
+    }
+}
+
 void Server::loop()
 {
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(m);
-            if (active.empty() && queue.empty()) {
-                if (stop) break;
+            if (active.empty() && ready.empty()) {
+                if (stop && queue.empty()) break;
                 cv_work.wait_for(lk, std::chrono::milliseconds(20));
                 continue;
             }
         }
         progress();
-        admit();
-        int rc = kOk;
-        for (int ph = 0; ph < 2 && rc == kOk; ++ph)
-            if (dirty[ph] || gstep[ph] > (1 << 29)) rc = rebuild(ph);
-        if (rc == kOk && members[0].empty() && members[1].empty()) {
-            std::this_thread::sleep_for(std::chrono::microseconds(100));  // only side-stream stages pending
+        const double now = now_ms();
+        bool starved = false;
+        for (int ph = 0; ph < 2; ++ph)
+            for (Request* r : members[ph])
+                if (!r->sel && r->paused_since >= 0 && now - r->paused_since > kMaxPauseMs) starved = true;
+        if (reselect || starved) select(now);
+        for (int ph = 0; ph < 2; ++ph)
+            if (gstep[ph] > (1 << 29)) {  // re-origin the table before the counters grow large
+                const int rc = rebuild(ph);
+                if (rc) fail_all(rc, "FD launch table");
+            }
+        if (table_of[0].empty() && table_of[1].empty()) {
+            std::unique_lock<std::mutex> lk(m);  // side-stream stages or admissions pending
+            cv_work.wait_for(lk, std::chrono::microseconds(100));
             continue;
         }
         // launches until the nearest phase end, at most kMaxLaunchesPerEpoch
         int E = kMaxLaunchesPerEpoch;
         for (int ph = 0; ph < 2; ++ph)
-            for (Request* r : members[ph]) E = std::min(E, r->launch0 + r->launches - glaunch[ph]);
+            for (Request* r : table_of[ph]) E = std::min(E, r->total - r->done);
         E = std::max(E, 1);
+        int rc = kOk;
         for (int k = 0; k < E && rc == kOk; ++k)
             for (int ph = 0; ph < 2 && rc == kOk; ++ph) {
-                if (members[ph].empty()) continue;
-                rc = ph == 0 ? launch_fwd_tb<float>(fd, djobs[0].as<FdJob<float>>(), dbmap[0].as<BlockMap>(),
+                if (table_of[ph].empty()) continue;
+                rc = ph == 0 ? launch_fwd_tb<float>(fd[0], djobs[0].as<FdJob<float>>(), dbmap[0].as<BlockMap>(),
                                                     nblocks[0], gstep[0], glaunch[0], nt_blk[0])
-                             : launch_adj_tb<float>(fd, djobs[1].as<FdJob<float>>(), dbmap[1].as<BlockMap>(),
+                             : launch_adj_tb<float>(fd[1], djobs[1].as<FdJob<float>>(), dbmap[1].as<BlockMap>(),
                                                     nblocks[1], gstep[1], glaunch[1], nt_blk[1]);
                 gstep[ph] += nt_blk[ph];
                 ++glaunch[ph];
                 ++n_launch;
             }
-        if (rc != kOk) {  // a launch failed: every request on the FD stream fails with it
-            std::vector<Request*> act;
-            {
-                std::lock_guard<std::mutex> lk(m);
-                act = active;
-            }
-            (void)hipStreamSynchronize(fd);
-            for (Request* r : act) fail(r, rc, "FD launch");
-            members[0].clear();
-            members[1].clear();
-            continue;
-        }
+        if (rc != kOk) { fail_all(rc, "FD launch"); continue; }
         for (int ph = 0; ph < 2; ++ph) {
+            if (table_of[ph].empty()) continue;
             std::vector<Request*> keep;
+            for (Request* r : table_of[ph]) r->done += E;
             for (Request* r : members[ph]) {
-                if (glaunch[ph] - r->launch0 < r->launches) { keep.push_back(r); continue; }
+                if (!r->sel || r->done < r->total) { keep.push_back(r); continue; }
                 const int erc = end_phase(r, ph);
                 if (erc) fail(r, erc, ph == 0 ? "dispersion stage" : "assembly stage");
-                dirty[ph] = true;
+                reselect = true;
             }
             members[ph].swap(keep);
-        }
-        // keep at most two epochs of launches queued ahead of the device
-        hipError_t he = hipEventRecord(epoch_ev[epoch], fd);
-        epoch_used[epoch] = true;
-        epoch ^= 1;
-        if (he == hipSuccess && epoch_used[epoch]) he = hipEventSynchronize(epoch_ev[epoch]);
-        if (he != hipSuccess) {
-            set_error("FD stream: %s", hipGetErrorString(he));
-            std::vector<Request*> act;
-            {
-                std::lock_guard<std::mutex> lk(m);
-                act = active;
+            // keep at most two epochs of launches queued ahead of the device
+            hipError_t he = hipEventRecord(epoch_ev[ph][epoch[ph]], fd[ph]);
+            epoch_used[ph][epoch[ph]] = true;
+            epoch[ph] ^= 1;
+            if (he == hipSuccess && epoch_used[ph][epoch[ph]]) he = hipEventSynchronize(epoch_ev[ph][epoch[ph]]);
+            if (he != hipSuccess) {
+                set_error("FD stream: %s", hipGetErrorString(he));
+                fail_all(kErrHip, "FD stream");
+                break;
             }
-            for (Request* r : act) fail(r, kErrHip, "FD stream");
-            members[0].clear();
-            members[1].clear();
         }
     }
 }
@@ -707,7 +801,7 @@ int swd_server_submit(swd_server* h, const swd_problem* p, const double* vp, con
         s.queue.push_back(r.get());
         s.reqs[r->id] = std::move(r);
     }
-    s.cv_work.notify_all();
+    s.cv_admit.notify_all();
     return kOk;
 }
 
@@ -744,14 +838,23 @@ int swd_server_wait(swd_server* h, long long ticket, double timeout_ms, double* 
     return rc;
 }
 
+int swd_server_set_wave_slots(swd_server* h, long long waves)
+{
+    if (!h || waves < 0) { set_error("bad wave slots"); return kErrArg; }
+    Server& s = h->s;
+    std::lock_guard<std::mutex> lk(s.m);
+    s.slots_override = waves;
+    return kOk;
+}
+
 int swd_server_stats(swd_server* h, long long* vals, int n)
 {
     if (!h || !vals) return kErrArg;
     Server& s = h->s;
     std::lock_guard<std::mutex> lk(s.m);
     const long long v[10] = {(long long)s.queue.size(), (long long)s.active.size(), s.n_done, s.n_failed,
-                             (long long)s.dev.used(), (long long)s.dev.capacity(), s.n_launch, s.peak_active,
-                             (long long)s.peak_bytes, s.n_rebuild};
+                             (long long)s.dev.used(), (long long)s.dev.capacity(), s.n_launch.load(), s.peak_active,
+                             (long long)s.peak_bytes, s.n_rebuild.load()};
     for (int i = 0; i < n && i < 10; ++i) vals[i] = v[i];
     return kOk;
 }

@@ -6,8 +6,9 @@ for one model.  The native server admits requests in arrival order while a
 slot is free and their buffers fit in the device budget, advances the shots
 of every admitted request in shared FD launches (requests join and leave the
 launch sequence independently) and runs each request's Radon / picking /
-adjoint-source and assembly stages on side streams.  ``max_active=1`` gives
-the per-request path: one request on the GPU at a time, in arrival order.
+adjoint-source and assembly stages on side streams.  ``max_active=1`` keeps
+one request on the GPU at a time; :func:`run_trace_per_request` is the plain
+per-request path (an Engine per request, one after another).
 """
 from __future__ import annotations
 
@@ -57,6 +58,10 @@ class Server:
         check(rc)
         return {"dk_vs": dk, "res_r": rr, "res_l": rl, "queued_ms": tm[0], "gpu_ms": tm[1], "total_ms": tm[2],
                 "submit_ms": tm[3]}
+
+    def set_wave_slots(self, waves: int) -> None:
+        """Resident waves the launch selection fills (0: the device's count)."""
+        check(load().swd_server_set_wave_slots(self.h, int(waves)))
 
     def stats(self) -> dict:
         v = np.zeros(10, dtype=np.int64)
@@ -151,10 +156,40 @@ def run_trace(server: Server, templates, trace) -> dict:
         results.append(server.wait(tickets[i]))
     th.join()
     wall = time.perf_counter() - t_start
-    lat = np.array([r["total_ms"] for r in results])
-    work = sum(cell_steps(subset(templates[ti][0], k)) for _, ti, k in trace)
     span_ms = max(r["submit_ms"] + r["total_ms"] for r in results) - min(r["submit_ms"] for r in results)
+    return _summary(trace, templates, [r["total_ms"] for r in results], span_ms, [r["queued_ms"] for r in results],
+                    wall, results)
+
+
+def _summary(trace, templates, lat_ms, span_ms, queued_ms, wall, results):
+    work = sum(cell_steps(subset(templates[ti][0], k)) for _, ti, k in trace)
+    lat = np.asarray(lat_ms)
     return {"requests": len(trace), "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "mean_queued_ms": float(np.mean([r["queued_ms"] for r in results])),
-            "req_per_s": len(trace) / (span_ms / 1000.0), "gcell_steps_per_s": work / (span_ms / 1000.0) / 1e9,
-            "wall_s": wall, "results": results}
+            "mean_queued_ms": float(np.mean(queued_ms)), "req_per_s": len(trace) / (span_ms / 1000.0),
+            "gcell_steps_per_s": work / (span_ms / 1000.0) / 1e9, "wall_s": wall, "results": results}
+
+
+def run_trace_per_request(templates, trace) -> dict:
+    """The plain per-request path on the same trace: each request, in arrival
+    order, through its own Engine (allocate, upload, gradient, free), one at
+    a time; latency from its arrival time to its result."""
+    t0 = time.perf_counter()
+    lat, queued, results = [], [], []
+    for ta, ti, k in trace:
+        now = time.perf_counter() - t0
+        if ta > now:
+            time.sleep(ta - now)
+        start = time.perf_counter() - t0
+        q, mod, cr_r, cr_l = templates[ti]
+        e = Engine(subset(q, k))
+        e.set_wavelet(mod["s"])
+        e.set_observed_picks(cr_r[:, :k], cr_l[:, :k])
+        e.set_model(mod["vp"], mod["vs"])
+        g = e.gradient()
+        e.close()
+        done = time.perf_counter() - t0
+        lat.append(1000.0 * (done - ta))
+        queued.append(1000.0 * (start - ta))
+        results.append(g)
+    wall = time.perf_counter() - t0
+    return _summary(trace, templates, lat, 1000.0 * wall, queued, wall, results)

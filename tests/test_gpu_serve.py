@@ -82,3 +82,18 @@ def test_one_request_at_a_time(templates):
     s.close()
     assert st["peak_active"] == 1 and st["completed"] == 4
     _check_same(templates, trace, out)
+
+
+def test_paused_requests_resume_exactly(templates):
+    """With room for one request's shots per launch the scheduler pauses the
+    others (least remaining work runs first, long-paused ones rejoin); paused
+    and resumed wavefields give the same results as uninterrupted solves."""
+    trace = poisson_trace(templates, rate=5000.0, n=6, seed=4, max_shots=6)
+    s = Server(hbm_bytes=8 << 30, max_active=6, nt_block=(3, 5))
+    s.set_wave_slots(1)
+    out = run_trace(s, templates, trace)
+    st = s.stats()
+    s.close()
+    assert st["completed"] == 6 and st["peak_active"] >= 2
+    assert st["rebuilds"] > 6                # selections changed while requests were in flight
+    _check_same(templates, trace, out)
