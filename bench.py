@@ -81,6 +81,23 @@ def cpu_baseline(prob, mod, seconds_hint=15.0):
                       f"FP32 C restatement of the MEX statement loops, {wall:.1f} s on {platform.processor() or 'host'}"}
 
 
+def measured_traffic(kname, shape, args):
+    """HBM bytes per launch of the kernel from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes over the same workload (swd_amd/fd_traffic.json, written
+    by ``python -m swd_amd.profile_fd summarize``), when the measured launch
+    shape is the one that ran; else None."""
+    path = os.path.join(ROOT, "swd_amd", "fd_traffic.json")
+    if not os.path.exists(path) or args.shots != 60 or args.nt != 6000:
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    k = kname.replace("_march", "_tb") if shape[0] > 1 else kname
+    rec = t.get("kernels", {}).get(k)
+    if rec is None or list(t.get("shapes", {}).get(k, [])) != list(shape):
+        return None
+    return int(rec["bytes_per_launch"])
+
+
 def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -255,8 +272,12 @@ def main():
         k = ka
         kname, kus, kbytes = f"fd_adj_{'tb' if k > 1 else 'march'}", a_us, k * fd_bytes_per_launch(prob, S, True)
     achieved = kbytes / (kus * 1e-6) / 1e9
+    kshape = [k, int(stage[-1]["fwd_band_rows" if kname.startswith("fd_fwd") else "adj_band_rows"])]
+    roof_traffic = measured_traffic(kname, kshape, args)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": kname,
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": roof_traffic,
+            "kernel": kname, "launch_shape": kshape,
+            "hbm_gbs_measured": (round(roof_traffic / (kus * 1e-6) / 1e9, 1) if roof_traffic else None),
             "kernel_us": round(kus, 2), "steps_per_launch": k, "algorithmic_bytes_per_launch": int(kbytes)}
 
     out = {"metric": "Gcell-steps/s FD fwd+adj (% HBM roofline), shots-grad/s; p50/p99 req latency",

@@ -230,6 +230,12 @@ int swd_engine_read(swd_engine *e, int what, void *dst, long long bytes);
  * march-kernel duration. */
 int swd_batch_gradient(swd_engine **engines, int n, double **dk_vs, double *kernel_us);
 
+/* HBM counter calibration (no reference counterpart): copies `bytes` between
+ * two device buffers `reps` times with a plain vector copy kernel; *ms = mean
+ * event-timed duration of one copy.  Its rocprofv3 FETCH_SIZE / WRITE_SIZE
+ * readings scale the FD kernels' readings to bytes. */
+int swd_calib_copy(long long bytes, int reps, double *ms);
+
 /* Number of Radon frequency bins (rows of the pick arrays) of a problem. */
 int swd_problem_npair(const swd_problem *p, int *npair);
 

@@ -1,0 +1,144 @@
+"""Workload and summaries of the FD kernel profiles.
+
+Run under rocprofv3 (one counter per pass):
+
+    rocprofv3 --pmc FETCH_SIZE -d <dir>/fetch --output-format csv -- python -m swd_amd.profile_fd run
+    rocprofv3 --pmc WRITE_SIZE -d <dir>/write --output-format csv -- python -m swd_amd.profile_fd run
+    rocprofv3 --kernel-trace --stats -d <dir>/trace --output-format csv -- python -m swd_amd.profile_fd run
+
+``run`` copies a known byte count with the calibration kernel, then computes
+one gradient of the production-shots workload (60 shots, 132 x 660 padded,
+nt 6000) with fixed forward / adjoint launch shapes.  ``summarize <dir>``
+reads the counter CSVs, scales the FD kernels' FETCH_SIZE / WRITE_SIZE by the
+copy kernel's (bytes known), and writes the per-launch HBM traffic of each
+time-blocked kernel to ``fd_traffic.json`` beside this file (read by the
+benchmark) and a markdown table to stdout.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TRAFFIC_JSON = os.path.join(HERE, "fd_traffic.json")
+COPY_BYTES = 1 << 30
+
+
+def _shape(s):
+    a, b = (int(x) for x in s.split(","))
+    return a, b
+
+
+def run(args):
+    import ctypes
+    from ._lib import check, load
+    from . import problems as P
+    from .engine import Engine
+    ms = ctypes.c_double(0)
+    check(load().swd_calib_copy(COPY_BYTES, args.copies, ctypes.byref(ms)))
+    print(f"copy of {COPY_BYTES} B: {ms.value:.3f} ms ({2 * COPY_BYTES / ms.value / 1e6:.0f} GB/s)", flush=True)
+    prob, mod = P.production_shots(nshots=args.shots, nt=args.nt)
+    e = Engine(prob)
+    e.set_wavelet(mod["s"])
+    e.set_observed_picks(np.ones((e.npair, e.nshots)), np.ones((e.npair, e.nshots)))
+    e.set_model(mod["vp"], mod["vs"])
+    (nf, hf), (na, ha) = _shape(args.fwd), _shape(args.adj)
+    e.set_time_block(nf, hf, phase=0)
+    e.set_time_block(na, ha, phase=1)
+    for _ in range(args.gradients):
+        e.gradient()
+    t = e.timings()
+    info = e.info()
+    e.close()
+    print(json.dumps({"fwd": [nf, hf], "adj": [na, ha], "fwd_kernel_us": t["fwd_kernel_us"],
+                      "adj_kernel_us": t["adj_kernel_us"], "nzbc": info["nzbc"], "nxbc": info["nxbc"],
+                      "shots": args.shots, "nt": args.nt}), flush=True)
+
+
+def _rows(d):
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise FileNotFoundError(f"no counter_collection.csv under {d}")
+    for f in files:
+        with open(f, newline="") as fh:
+            yield from csv.DictReader(fh)
+
+
+def _per_kernel(d, counter):
+    """Mean counter value (KiB) per dispatch of each kernel, summed over the
+    dimensions rocprofv3 reports it in."""
+    per = {}
+    for r in _rows(d):
+        if r.get("Counter_Name") != counter:
+            continue
+        key = (r["Kernel_Name"], r.get("Dispatch_Id") or r.get("Correlation_Id"))
+        per[key] = per.get(key, 0.0) + float(r["Counter_Value"])
+    out = {}
+    for (k, _), v in per.items():
+        out.setdefault(k, []).append(v)
+    return {k: (float(np.mean(v)), len(v)) for k, v in out.items()}
+
+
+def _short(name):
+    for k in ("fd_fwd_tb", "fd_adj_tb", "fd_fwd_march", "fd_adj_march", "calib_copy"):
+        if k in name:
+            return k
+    return None
+
+
+def summarize(args):
+    fetch = _per_kernel(os.path.join(args.dir, "fetch"), "FETCH_SIZE")
+    write = _per_kernel(os.path.join(args.dir, "write"), "WRITE_SIZE")
+    cf = [v for k, v in fetch.items() if _short(k) == "calib_copy"]
+    cw = [v for k, v in write.items() if _short(k) == "calib_copy"]
+    if not cf or not cw:
+        raise RuntimeError("calibration copy kernel missing from the counter files")
+    kib = COPY_BYTES / 1024.0
+    f_fetch, f_write = kib / cf[0][0], kib / cw[0][0]
+    res = {"copy_bytes": COPY_BYTES, "fetch_scale": f_fetch, "write_scale": f_write,
+           "copy_fetch_kib": cf[0][0], "copy_write_kib": cw[0][0], "kernels": {}}
+    lines = ["| kernel | dispatches | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch | HBM bytes/dispatch |",
+             "|---|---|---|---|---|"]
+    for k, (fv, n) in sorted(fetch.items()):
+        s = _short(k)
+        if s is None or s == "calib_copy":
+            continue
+        wv = write.get(k, (0.0, 0))[0]
+        b = 1024.0 * (fv * f_fetch + wv * f_write)
+        res["kernels"][s] = {"fetch_kib": fv, "write_kib": wv, "bytes_per_launch": b, "dispatches": n,
+                             "name": k}
+        lines.append(f"| {s} | {n} | {fv:.0f} | {wv:.0f} | {b / 1e6:.1f} MB |")
+    if args.shapes:
+        res["shapes"] = json.loads(args.shapes)
+    with open(TRAFFIC_JSON, "w") as fh:
+        json.dump(res, fh, indent=1)
+    print(f"calibration: copy of {COPY_BYTES} B reads {cf[0][0]:.0f} KiB (scale {f_fetch:.4f}), "
+          f"writes {cw[0][0]:.0f} KiB (scale {f_write:.4f})")
+    print("\n".join(lines))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    r = sub.add_parser("run")
+    r.add_argument("--fwd", default="5,19", help="forward steps per launch, band rows")
+    r.add_argument("--adj", default="4,33", help="adjoint steps per launch, band rows")
+    r.add_argument("--shots", type=int, default=60)
+    r.add_argument("--nt", type=int, default=6000)
+    r.add_argument("--gradients", type=int, default=1)
+    r.add_argument("--copies", type=int, default=5)
+    s = sub.add_parser("summarize")
+    s.add_argument("dir")
+    s.add_argument("--shapes", default='{"fd_fwd_tb": [5, 19], "fd_adj_tb": [4, 33]}')
+    a = ap.parse_args(argv)
+    (run if a.cmd == "run" else summarize)(a)
+
+
+if __name__ == "__main__":
+    main()
