@@ -81,6 +81,17 @@ def cpu_baseline(prob, mod, seconds_hint=15.0):
                       f"FP32 C restatement of the MEX statement loops, {wall:.1f} s on {platform.processor() or 'host'}"}
 
 
+def copy_peak(nbytes=1 << 30, reps=10):
+    """HBM rate of a plain float4 device copy of 1 GiB (read + write bytes per
+    second), measured now: the practical ceiling beside the 8 TB/s figure."""
+    import ctypes
+    from swd_amd._lib import check, load
+    ms = ctypes.c_double(0)
+    check(load().swd_calib_copy(nbytes, 2, ctypes.byref(ms)))
+    check(load().swd_calib_copy(nbytes, reps, ctypes.byref(ms)))
+    return 2.0 * nbytes / (ms.value * 1e-3) / 1e9
+
+
 def measured_traffic(kname, shape, args):
     """HBM bytes per launch of the kernel from the rocprofv3 FETCH_SIZE /
     WRITE_SIZE passes over the same workload (swd_amd/fd_traffic.json, written
@@ -159,7 +170,55 @@ def uneven_leg(steps=2):
                     "batch gathers of shot 1 of each model vs the FP64 GPU MEX path"}
 
 
-def serving_leg(rates=(2.0, 10.0, 20.0), n=32, seed=0):
+PEAK_MFMA_FP32_TFLOPS = 157.3  # MI355X dense FP32 MFMA
+
+
+def dispersion_leg(betas=(5.0, 10.0, 20.0), reps=3):
+    """The dispersion path alone: 60 chirp gathers of 6000 x 600 (dt 8e-5,
+    0.5 m traces, phase velocity 250 + 300 exp(-f/20) m/s, 10% band-limited
+    noise), Radon over 151 frequencies x 701 velocities for every (shot,
+    side) window, picks, then adjoint sources for each beta.  FLOPs: the DFT
+    at the bins and the inverse DFT are real GEMMs of 2 x (2 npair) x nt x
+    ng per shot; each slant stack is 8 x np x npair x (total window length)."""
+    from swd_amd import problems as P
+    from swd_amd.dispersion import DispersionBatch
+    prob, _ = P.production_shots(nshots=60)
+    g = P.chirp_gathers(nshots=60, nt=prob["nt"], ng=prob["ng"], dt=prob["dt"], dx=prob["ds"], sx=prob["sx"])
+    D = DispersionBatch(prob)
+    D.set_gathers(g)
+    D.set_observed_picks(np.ones((D.npair, 60)), np.ones((D.npair, 60)))
+    D.pick()
+    pick_ms = min(D.pick()["ms"] for _ in range(reps))
+    r = D.pick()
+    adj_ms = {}
+    for b in betas:
+        D.adjoint(b, want=False)
+        adj_ms[f"{b:g}"] = round(min(D.adjoint(b, want=False)[1] for _ in range(reps)), 3)
+    D.close()
+    nt, ng, npair, S = prob["nt"], prob["ng"], D.npair, 60
+    dft = 2.0 * (2 * npair) * min(nt, D.ccn) * ng * S
+    slant = 8.0 * D.np * npair * D.sum_l
+    f = prob["fmin"] + prob["df"] * np.arange(npair)
+    c = P.chirp_phase_velocity(f)
+    band = (f >= 15) & (f <= 70)
+    # picks are cr_pre = index + vmin, one bin above the axis velocity; judged on windows of >= 200 traces
+    # (100 m), where the aperture resolves the chirp (offset 600 >= ng: right window ng - sx + 1, left sx)
+    sx = np.asarray(prob["sx"])
+    ok = [np.abs((r["cr_r"][:, s] - 1.0) - c)[band] <= 1.0 for s in range(S) if ng - sx[s] + 1 >= 200] + \
+         [np.abs((r["cr_l"][:, s] - 1.0) - c)[band] <= 1.0 for s in range(S) if sx[s] >= 200]
+    amean = float(np.mean(list(adj_ms.values())))
+    return {"pick_ms": round(pick_ms, 3), "adjoint_ms": adj_ms, "shots": S,
+            "shots_per_s": round(S / ((pick_ms + amean) * 1e-3), 1),
+            "mfma": {"forward_gflop": round((dft + slant) / 1e9, 1), "adjoint_gflop": round((dft + slant) / 1e9, 1),
+                     "forward_tflops": round((dft + slant) / (pick_ms * 1e-3) / 1e12, 2),
+                     "adjoint_tflops": round((dft + slant) / (amean * 1e-3) / 1e12, 2),
+                     "peak_fp32_tflops": PEAK_MFMA_FP32_TFLOPS},
+            "picks_within_1_bin_of_c_15_70hz": round(float(np.mean(np.concatenate(ok))), 4),
+            "pick_columns_judged": len(ok),
+            "what": "times include the small kernels of each stage (gather, |mm|, picks, softargmax, placement)"}
+
+
+def serving_leg(rates=(2.0, 5.0, 10.0, 20.0), n=32, seed=0):
     """The serving loop: Poisson arrivals of forward + gradient requests, each
     a random model of the uneven set with 1..32 shots.  "batched" is the
     continuous-batching server (HBM-budget admission, up to 16 requests on the
@@ -216,6 +275,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven multi-model batch leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
+    ap.add_argument("--no-dispersion", action="store_true", help="skip the dispersion-path leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the shot-parallel multi-GPU "
@@ -265,6 +325,7 @@ def main():
     # one launch advances `k` time steps, so its algorithmic bytes are k steps' worth
     f_us, a_us = float(np.mean(fwd_us)), float(np.mean(adj_us))
     kf, ka = int(stage[-1]["fwd_steps_per_launch"]), int(stage[-1]["adj_steps_per_launch"])
+    copy_gbs = copy_peak()
     if f_us * nt / kf >= a_us * (nt - 1) / ka:
         k = kf
         kname, kus, kbytes = f"fd_fwd_{'tb' if k > 1 else 'march'}", f_us, k * fd_bytes_per_launch(prob, S, False)
@@ -278,6 +339,8 @@ def main():
             "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": roof_traffic,
             "kernel": kname, "launch_shape": kshape,
             "hbm_gbs_measured": (round(roof_traffic / (kus * 1e-6) / 1e9, 1) if roof_traffic else None),
+            "copy_peak_gbs": round(copy_gbs, 1),
+            "hbm_frac_of_copy_peak": (round(roof_traffic / (kus * 1e-6) / 1e9 / copy_gbs, 3) if roof_traffic else None),
             "kernel_us": round(kus, 2), "steps_per_launch": k, "algorithmic_bytes_per_launch": int(kbytes)}
 
     out = {"metric": "Gcell-steps/s FD fwd+adj (% HBM roofline), shots-grad/s; p50/p99 req latency",
@@ -288,6 +351,7 @@ def main():
            "config": {"workload": "single-model production shots: 60 shots, 100x600 cells (132x660 padded), "
                                   "nt=6000, dt=8e-5, 600 receivers; Radon 151 freq x 701 vel; beta=10",
                       "shots": S, "nt": nt, "grid": [info["nzbc"], info["nxbc"]], "parallelism": "1 GPU, shot batch"},
+           "interior_gcell_steps_per_s": round(S * prob["nz"] * prob["nx"] * (nt + nt - 1) / (wall / args.steps) / 1e9, 3),
            "shots_grad_per_s": round(S / (wall / args.steps), 3),
            "gradient_latency_ms": {"p50": round(1000 * float(np.percentile(per_step, 50)), 2),
                           "p99": round(1000 * float(np.percentile(per_step, 99)), 2)},
@@ -316,6 +380,9 @@ def main():
         e2.close()
     eng.close()
     progress(f"headline: {value:.1f} Gcell-steps/s")
+    if not args.no_dispersion:
+        out["dispersion"] = dispersion_leg()
+        progress(f"dispersion: picks {out['dispersion']['pick_ms']} ms for 60 shots")
     if not args.no_uneven:
         out["uneven_batch"] = uneven_leg()
     if not args.no_serving:

@@ -236,6 +236,34 @@ int swd_batch_gradient(swd_engine **engines, int n, double **dk_vs, double *kern
  * readings scale the FD kernels' readings to bytes. */
 int swd_calib_copy(long long bytes, int reps, double *ms);
 
+/*
+ * The dispersion path alone, batched over every (shot, side) column of
+ * nshots gathers (the per-shot RTr/RTl + LHDispPick of SWD.m:173-185 and
+ * RTrAD/RTlAD + ADWDgrad_1 of SWD.m:190, with the branch guards of
+ * SWD.m:175,181), on gathers the caller provides.  Of the problem struct it
+ * uses the Radon, pick and window fields: nt, dt, ng, nshots, is_index, ns,
+ * df, fmin, fmax, vmin, vmax, ds, offset, m, M, ini, w, beta; the FD fields
+ * are ignored.
+ * swd_dispersion_info: vals[0..7] = npair, np, lf, nf, ccn, longest window,
+ * total window length and number of enabled (shot, side) columns.
+ * swd_dispersion_set_gathers: [shot][t][g] float (nshots x nt x ng, g fastest).
+ * swd_dispersion_pick: picks cr_r / cr_l (npair x nshots column-major, in
+ * m/s, cr_pre = idx + vmin), misfits res_r / res_l (nshots); *ms = device
+ * time of Radon + picks.  SWD_ERR_PICK when an enabled column's pick keeps
+ * fewer than two columns.
+ * swd_dispersion_adjoint: adjoint sources [shot][t][g] of the last picks
+ * against the observed picks (beta <= 0: the problem's beta); *ms = device
+ * time.
+ */
+typedef struct swd_dispersion swd_dispersion;
+int swd_dispersion_create(const swd_problem *p, swd_dispersion **out);
+int swd_dispersion_destroy(swd_dispersion *d);
+int swd_dispersion_info(swd_dispersion *d, long long *vals, int n);
+int swd_dispersion_set_gathers(swd_dispersion *d, const float *gathers);
+int swd_dispersion_set_observed_picks(swd_dispersion *d, const double *cr0_r, const double *cr0_l);
+int swd_dispersion_pick(swd_dispersion *d, double *cr_r, double *cr_l, double *res_r, double *res_l, double *ms);
+int swd_dispersion_adjoint(swd_dispersion *d, double beta, float *adj, double *ms);
+
 /* Number of Radon frequency bins (rows of the pick arrays) of a problem. */
 int swd_problem_npair(const swd_problem *p, int *npair);
 

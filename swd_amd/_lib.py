@@ -68,6 +68,13 @@ SIGNATURES = {
     "swd_engine_read": (_I, [_P, _I, _P, ctypes.c_longlong]),
     "swd_batch_gradient": (_I, [_P, _I, _P, _P]),
     "swd_problem_npair": (_I, [_P, _P]),
+    "swd_dispersion_create": (_I, [_P, _P]),
+    "swd_dispersion_destroy": (_I, [_P]),
+    "swd_dispersion_info": (_I, [_P, _P, _I]),
+    "swd_dispersion_set_gathers": (_I, [_P, _P]),
+    "swd_dispersion_set_observed_picks": (_I, [_P, _P, _P]),
+    "swd_dispersion_pick": (_I, [_P, _P, _P, _P, _P, _P]),
+    "swd_dispersion_adjoint": (_I, [_P, ctypes.c_double, _P, _P]),
     "swd_calib_copy": (_I, [ctypes.c_longlong, _I, _P]),
     "swd_server_create": (_I, [ctypes.c_longlong, _I, _I, _I, _P]),
     "swd_server_destroy": (_I, [_P]),

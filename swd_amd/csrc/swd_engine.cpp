@@ -32,6 +32,30 @@ int problem_geometry(const swd_problem& p, DispGeom& g)
     return disp_geometry(g);
 }
 
+int build_disp_plan(const swd_problem& p, const std::vector<int>& is_index, hipStream_t st, DispPlan& disp)
+{
+    DispGeom g;
+    int rc = problem_geometry(p, g);
+    if (rc) return rc;
+    if (p.M == 0) { set_error("M must be non-zero"); return kErrArg; }
+    if (p.ini < 1 || p.ini > g.npair) { set_error("ini %d outside 1..%d", p.ini, g.npair); return kErrArg; }
+    if ((int)is_index.size() != p.nshots) { set_error("is_index has %zu entries for %d shots", is_index.size(), p.nshots); return kErrArg; }
+    // branch guards of SWD.m:175,181 (right: is <= ns - floor(m/M) - round(w/M); left: is >= round(w/M) + floor(m/M) + 1)
+    const int ns = p.ns > 0 ? p.ns : p.nshots;
+    auto mr = [](double x) { return (int)(x >= 0 ? std::floor(x + 0.5) : -std::floor(-x + 0.5)); };
+    std::vector<DispCol> cols(2 * p.nshots);
+    for (int side = 0; side < 2; ++side)
+        for (int s = 0; s < p.nshots; ++s) {
+            const int is = is_index[s];
+            DispCol c = window_col(side, is, p.ng, p.offset, p.m, p.M, s);
+            const bool guard = side == 0 ? is <= ns - (int)std::floor((double)p.m / p.M) - mr(p.w / p.M)
+                                         : is >= mr(p.w / p.M) + (int)std::floor((double)p.m / p.M) + 1;
+            if (!guard) c.enabled = 0;
+            cols[side * p.nshots + s] = c;
+        }
+    return disp.init(g, cols, st);
+}
+
 int Engine::create(const swd_problem* prob, hipStream_t stream)
 {
     if (!prob) { set_error("null problem"); return kErrArg; }
@@ -75,27 +99,10 @@ int Engine::create(const swd_problem* prob, hipStream_t stream)
         own_stream = true;
     }
     tz = fd_tile_rows();
-    // dispersion plan
-    DispGeom g;
-    int rc = problem_geometry(p, g);
+    int rc = build_disp_plan(p, is_index, st, disp);
     if (rc) return rc;
-    if (p.M == 0) { set_error("M must be non-zero"); return kErrArg; }
-    npair = g.npair;
-    np = g.np;
-    if (p.ini < 1 || p.ini > npair) { set_error("ini %d outside 1..%d", p.ini, npair); return kErrArg; }
-    const int ns = p.ns > 0 ? p.ns : p.nshots;
-    auto mr = [](double x) { return (int)(x >= 0 ? std::floor(x + 0.5) : -std::floor(-x + 0.5)); };
-    std::vector<DispCol> cols(2 * p.nshots);
-    for (int side = 0; side < 2; ++side)
-        for (int s = 0; s < p.nshots; ++s) {
-            const int is = is_index[s];
-            DispCol c = window_col(side, is, p.ng, p.offset, p.m, p.M, s);
-            const bool guard = side == 0 ? is <= ns - (int)std::floor((double)p.m / p.M) - mr(p.w / p.M)
-                                         : is >= mr(p.w / p.M) + (int)std::floor((double)p.m / p.M) + 1;
-            if (!guard) c.enabled = 0;
-            cols[side * p.nshots + s] = c;
-        }
-    if ((rc = disp.init(g, cols, st))) return rc;
+    npair = disp.g.npair;
+    np = disp.g.np;
     // device buffers
     const size_t S = p.nshots, ntr = (size_t)p.nt * p.ng;
     if ((rc = coefs.alloc(6 * plane * sizeof(float))) || (rc = vs_d.alloc(splane * sizeof(double))) ||

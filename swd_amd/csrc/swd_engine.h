@@ -15,6 +15,10 @@ namespace swd {
 // Radon geometry (frequency bins, velocities) of a problem.
 int problem_geometry(const swd_problem& p, DispGeom& g);
 
+// Batched dispersion plan of a problem's (shot, side) columns: the RTr/RTl
+// windows of each shot index and the branch guards of SWD.m:175,181.
+int build_disp_plan(const swd_problem& p, const std::vector<int>& is_index, hipStream_t st, DispPlan& disp);
+
 int launch_assemble_gradient(hipStream_t st, const double* imgs, int nshots, int nz, int nx, const double* vs,
                              double* dk, double* gcl, double* gcm, double* gil);
 

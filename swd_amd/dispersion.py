@@ -117,3 +117,66 @@ def ADWDgrad_1(nt, ng, ns, npair, is_, w, m, M, SoftArgNorm, grad_outputr, grad_
                                ptr(mll), posl, Ll, int(ref["np"]), float(ref["vmin"]), float(ref["df"]),
                                float(ref["ds"]), int(ref["lf"]), int(ref["ccn"]), ptr(out)))
     return out
+
+
+class DispersionBatch:
+    """The dispersion path alone on caller-provided gathers, batched over
+    every (shot, side) column (swd_dispersion_* in include/swd_abi.h): RTr /
+    RTl + LHDispPick, then RTrAD / RTlAD + ADWDgrad_1, as SWD.m:173-190 runs
+    them per shot."""
+
+    def __init__(self, problem: dict):
+        import ctypes
+        from .engine import make_problem
+        self.problem = dict(problem)
+        p, self._keep = make_problem(self.problem)
+        h = ctypes.c_void_p()
+        check(load().swd_dispersion_create(ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+        v = np.zeros(8, dtype=np.int64)
+        check(load().swd_dispersion_info(self.h, ptr(v), 8))
+        self.npair, self.np, self.lf, self.nf, self.ccn, self.lmax, self.sum_l, self.ncol = (int(x) for x in v)
+        self.nshots = int(np.size(self.problem["sx"]))
+
+    def set_gathers(self, gathers):
+        """gathers: (nshots, nt, ng) float32, g fastest."""
+        g = np.ascontiguousarray(gathers, dtype=np.float32)
+        q = self.problem
+        if g.shape != (self.nshots, q["nt"], q["ng"]):
+            raise ValueError(f"gathers must be {(self.nshots, q['nt'], q['ng'])}, got {g.shape}")
+        check(load().swd_dispersion_set_gathers(self.h, ptr(g)))
+
+    def set_observed_picks(self, cr0_r, cr0_l):
+        check(load().swd_dispersion_set_observed_picks(self.h, ptr(fortran(cr0_r, np.float64)),
+                                                        ptr(fortran(cr0_l, np.float64))))
+
+    def pick(self):
+        """Picks and misfits of the resident gathers: dict cr_r, cr_l
+        (npair x nshots), res_r, res_l (nshots), ms (device time)."""
+        import ctypes
+        cr = np.zeros((self.npair, self.nshots), order="F")
+        cl = np.zeros((self.npair, self.nshots), order="F")
+        rr, rl = np.zeros(self.nshots), np.zeros(self.nshots)
+        ms = ctypes.c_double(0)
+        check(load().swd_dispersion_pick(self.h, ptr(cr), ptr(cl), ptr(rr), ptr(rl), ctypes.byref(ms)))
+        return {"cr_r": cr, "cr_l": cl, "res_r": rr, "res_l": rl, "ms": ms.value}
+
+    def adjoint(self, beta: float = 0.0, want=True):
+        """Adjoint sources (nshots, nt, ng) of the last picks; ms = device time."""
+        import ctypes
+        q = self.problem
+        out = np.zeros((self.nshots, q["nt"], q["ng"]), dtype=np.float32) if want else None
+        ms = ctypes.c_double(0)
+        check(load().swd_dispersion_adjoint(self.h, float(beta), ptr(out) if want else None, ctypes.byref(ms)))
+        return out, ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().swd_dispersion_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

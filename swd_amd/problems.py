@@ -87,3 +87,53 @@ def uneven_models(seed=20261203, shots=16, sizes=(200, 300, 450, 600, 800, 900, 
         out.append((prob, {"vs_true": vs, "vp_true": 1.73 * vs, "vs": vs_cur, "vp": 1.73 * vs_cur,
                            "s": ricker(f0, dt, nt)}))
     return out
+
+
+def chirp_phase_velocity(f):
+    """Phase velocity of the synthetic dispersive wave, c(f) = 250 + 300 exp(-f/20) m/s."""
+    return 250.0 + 300.0 * np.exp(-np.asarray(f, dtype=np.float64) / 20.0)
+
+
+def chirp_gathers(nshots=60, nt=6000, ng=600, dt=8e-5, dx=0.5, sx=None, noise=0.1, seed=20261203,
+                  fmin=5.0, fmax=80.0, df=0.5):
+    """Dispersion-path gathers [shot][t][g] (float32): Gaussian noise band-
+    limited to fmin..fmax plus a dispersive surface wave with phase velocity
+    chirp_phase_velocity(f), radiating from each shot's source trace x_s
+    (1-based trace sx[s]) with spectrum exp(-i 2 pi f (t0 + |x - x_s| / c(f))).
+    The wave is synthesised on a 1/df-second grid (the Radon's own frequency
+    spacing) and cut to the nt-sample record, so arrivals beyond the record
+    are absent as in a real gather.  noise = noise RMS / signal RMS."""
+    rng = np.random.default_rng(seed)
+    if sx is None:
+        sx = 1 + 10 * np.arange(nshots)
+    N = max(nt, int(round(1.0 / df / dt)))
+    f = np.fft.rfftfreq(N, dt)
+    band = (f >= fmin) & (f <= fmax)
+    taper = np.zeros_like(f)
+    w = 2.0  # cosine tapers over 2 Hz at both band edges
+    taper[band] = 1.0
+    lo = (f >= fmin) & (f < fmin + w)
+    hi = (f > fmax - w) & (f <= fmax)
+    taper[lo] = 0.5 - 0.5 * np.cos(np.pi * (f[lo] - fmin) / w)
+    taper[hi] = 0.5 - 0.5 * np.cos(np.pi * (fmax - f[hi]) / w)
+    c = np.where(band, chirp_phase_velocity(np.maximum(f, 1e-6)), 1.0)
+    # the wave depends on the offset |x - x_s| only: synthesise every offset once
+    t0 = 0.02  # onset delay (s)
+    offs = np.arange(ng) * dx
+    sig_by_off = np.empty((nt, ng))
+    for j0 in range(0, ng, 100):  # offset blocks keep the complex spectra small
+        o = offs[j0:j0 + 100]
+        ph = np.exp(-2j * np.pi * f[:, None] * (o[None, :] / c[:, None] + t0))
+        sig_by_off[:, j0:j0 + 100] = np.fft.irfft(taper[:, None] * ph, n=N, axis=0)[:nt]
+    # noise: band-limited on the record's own grid
+    fr = np.fft.rfftfreq(nt, dt)
+    nband = ((fr >= fmin) & (fr <= fmax)).astype(np.float64)
+    out = np.empty((nshots, nt, ng), dtype=np.float32)
+    g_idx = np.arange(ng)
+    for s in range(nshots):
+        sig = sig_by_off[:, np.abs(g_idx - (int(sx[s]) - 1))]
+        nz = np.fft.irfft(nband[:, None] * np.fft.rfft(rng.standard_normal((nt, ng)), axis=0), n=nt, axis=0)
+        sr = np.sqrt(np.mean(sig ** 2))
+        nr = np.sqrt(np.mean(nz ** 2))
+        out[s] = (sig + noise * sr / max(nr, 1e-30) * nz).astype(np.float32)
+    return out
