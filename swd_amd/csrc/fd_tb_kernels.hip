@@ -85,6 +85,14 @@ struct RowLoad {
     }
 };
 
+// Snapshot and image values of one imaging row for this lane, loaded one
+// row ahead like the state (raw snapshot fields, current FP64 images).
+template <typename T>
+struct ImgRow {
+    T s0, s1, s2, s3;
+    double m0, m1, m2;
+};
+
 template <typename T, int NT>
 __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJob<T>* __restrict__ jobs,
                                                                       const BlockMap* __restrict__ bmap, int step,
@@ -310,6 +318,32 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
         inj[l] = (valid[l] && gi >= 0) ? J.dt * J.src[(size_t)itl[l] * J.ng + gi] : T(0);
     }
     const bool xint = x >= J.nbc && x < J.nbc + J.nx;
+    const bool img_lane = out && xint;
+    // Imaging levels of this launch.  With exactly one (the usual case,
+    // in_wf >= NT) its snapshot and image values are prefetched one row
+    // ahead with the state; otherwise every imaging level loads inline.
+    int limg = -1, nimg = 0;
+#pragma unroll
+    for (int l = 0; l < NT; ++l)
+        if (valid[l] && (itl[l] % in_wf) == 0) { limg = l; ++nimg; }
+    const bool pf_img = nimg == 1;
+    const size_t iplane = (size_t)nzi * nxi;
+    const gptr<const T> snb = snap + (pf_img ? (size_t)(itl[limg] / in_wf) * nsf * iplane : 0);
+    auto img_load = [&](ImgRow<T>& q, const int i) {
+        const int r = i - limg - 1;
+        if (pf_img && r >= z0 && r < z1 && r > pt && r <= pt + nzi && img_lane) {
+            const size_t io = (size_t)(r - pt - 1) * nxi + (x - nbc);
+            const gptr<const T> sn = snb + io;
+            q.s0 = sn[0];
+            q.s1 = sn[iplane];
+            q.s2 = sn[2 * iplane];
+            if (nsf == 4) q.s3 = sn[3 * iplane];
+            const gptr<const double> im = img + io;
+            q.m0 = im[0];
+            q.m1 = im[iplane];
+            q.m2 = im[2 * iplane];
+        }
+    };
 
     // carried per level: previous input row (stress, its coefficient products,
     // old ww), previous outputs u_l(r), w_l(r-1)
@@ -322,7 +356,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
     for (int k = 0; k <= NT; ++k) ring[k] = CoefRow<T>{T(0), T(0), T(0), T(0), T(0), T(0)};
 
     const int i0 = z0 - NT - 1, i1 = z1 - 1 + NT;
-    auto row = [&](const RowLoad<T>& cur, const int i) {
+    auto row = [&](const RowLoad<T>& cur, const ImgRow<T>& icur, const int i) {
         T ixx = cur.xx, izz = cur.zz, ixz = cur.xz, iu = cur.uu, iw = cur.ww;
 #pragma unroll
         for (int k = NT; k > 0; --k) ring[k] = ring[k - 1];
@@ -351,8 +385,23 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
             const T oxx = cell_adj_stress(tc, cxx[l], dtx, u_rt - u_r);
             const T ozz = (r == pt) ? T(0) : cell_adj_stress(tc, czz[l], dtx, w_r - cwo[l]);
             const T oxz = cell_adj_shear(tc, cxz[l], dtx, u_q - u_r, w_r - w_l);
-            if (out && valid[l] && r >= z0 && r < z1 && (itl[l] % in_wf) == 0 && xint && r > pt &&
-                r <= pt + nzi) {
+            if (pf_img) {
+                if (l == limg && r >= z0 && r < z1 && r > pt && r <= pt + nzi && img_lane) {
+                    // prefetched snapshot and image values of this row
+                    double Fux, Fwz, Fsh;
+                    if (nsf == 3) {
+                        Fux = icur.s0; Fwz = icur.s1; Fsh = icur.s2;
+                    } else {
+                        Fux = icur.s0; Fwz = icur.s3; Fsh = (double)(icur.s1 + icur.s2);
+                    }
+                    const double X = oxx, Z = ozz, XZ = oxz;
+                    const gptr<double> im = img + (size_t)(r - pt - 1) * nxi + (x - nbc);
+                    im[0] = icur.m0 + (Fux + Fwz) * (X + Z);
+                    im[iplane] = icur.m1 + 2.0 * (X * Fux + Z * Fwz + 0.5 * XZ * Fsh);
+                    im[2 * iplane] = icur.m2 + (Fux + Fwz) * (Fux + Fwz);
+                }
+            } else if (out && valid[l] && r >= z0 && r < z1 && (itl[l] % in_wf) == 0 && xint && r > pt &&
+                       r <= pt + nzi) {
                 const size_t plane = (size_t)nzi * nxi;
                 const size_t io = (size_t)(r - pt - 1) * nxi + (x - nbc);
                 const gptr<const T> sn = snap + (size_t)(itl[l] / in_wf) * nsf * plane + io;
@@ -387,13 +436,17 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
     // Rows are loaded one step ahead into two alternating buffers, so the
     // loads of row i+1 are in flight while row i is processed.
     RowLoad<T> A, B;
+    ImgRow<T> IA, IB;
     A.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i0, nzbc, rowb, xb);
+    img_load(IA, i0);
     for (int i = i0; i <= i1; i += 2) {
         B.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i + 1, nzbc, rowb, xb);
-        row(A, i);
+        img_load(IB, i + 1);
+        row(A, IA, i);
         if (i + 1 > i1) break;
         A.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i + 2, nzbc, rowb, xb);
-        row(B, i + 1);
+        img_load(IA, i + 2);
+        row(B, IB, i + 1);
     }
 }
 
