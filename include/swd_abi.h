@@ -223,8 +223,38 @@ int swd_engine_info(swd_engine *e, long long *vals, int n);
 int swd_engine_timings(swd_engine *e, double *ms, int n);
 /* Copy a device buffer to host: 0 gathers [shot][t][g] f32, 1 adjoint sources
  * [shot][t][g] f32, 2 {dk_vs, sum cl_img, sum cm_img, sum illum} nz x nx f64,
- * 3 spectra [col][npair][np] f32, 4 per-shot images [shot][3][nz][nx] f64. */
+ * 3 spectra [col][npair][np] f32, 4 per-shot images [shot][3][nz][nx] f64,
+ * 5 the smoothed dk_vs of the last swd_engine_iterate, nz x nx f64. */
 int swd_engine_read(swd_engine *e, int what, void *dst, long long bytes);
+/*
+ * One SWD inversion iteration on the device (SWD.m:171-335 for one k):
+ * gradient of the engine's model, dk_vs = smooth2a(dk_vs, smooth_z,
+ * smooth_x), alpha = ||vs|| / ||dk_vs|| * pur, the line search of
+ * forward-only batches (halve f1 while the right-side misfit grows, else try
+ * 2 f1), the parabola step gama, and vs = clip(vs + alpha gama dk_vs,
+ * vsmin, vsmax) with the reference's fallback to the better trial.  The
+ * model stays on the device (swd_engine_read_model reads it); vp is the one
+ * given to swd_engine_set_model.  f1, f2, res2 carry the script's state
+ * between iterations (f1 = 0.5 at the first; f2 and res2 are used only when
+ * the halving loop does not run).  The offset schedule of SWD.m:326-333
+ * (rebuilding windows and observed picks) is the caller's.
+ */
+typedef struct swd_iteration {
+    double pur, vsmin, vsmax;  /* SWD.m:76,99 */
+    int smooth_z, smooth_x;    /* SWD.m:116 */
+    double f1, f2, res2;       /* carried between iterations */
+    double residual;           /* out: residual(k), mean right-side misfit of the gradient pass */
+    double res0;               /* out: misfit of the accepted model */
+    double alpha, gama;        /* out: step scale and accepted step factor */
+    double res1, res3;         /* out: misfits of the last halving / parabola trials */
+    int forwards;              /* out: forward-only batches run */
+} swd_iteration;
+int swd_engine_iterate(swd_engine *e, swd_iteration *it);
+int swd_engine_read_model(swd_engine *e, double *vs);
+/* smooth2a (core/smooth2a.m:52-76) on the device: (2 nr + 1) x (2 nc + 1) box
+ * mean with truncated edges, NaNs ignored; nz x nx column-major. */
+int swd_smooth2a(const double *in, int nz, int nx, int nr, int nc, double *out);
+
 /* Gradients of n engines (models of different sizes and nt) with shared FD
  * launches; dk_vs[i] is nz_i x nx_i.  kernel_us[2] = sampled forward/adjoint
  * march-kernel duration. */

@@ -49,6 +49,9 @@ PARITY = {
     "ADWDgrad_1": "finite differences of the soft-argmax misfit, slant-stack dot-product test, "
                   "one-sided real(ifft) closed form",
     "smooth2a": "box mean with truncated edges, brute force",
+    "swd_line_search": "quadratic misfit in the step factor: the parabola step lands on the exact minimum; "
+                       "the halving branch, the doubling branch, the fallback to the better trial and the "
+                       "clipping each checked on closed-form misfits",
 }
 
 
@@ -555,6 +558,64 @@ def smooth2a(A, Nr: int, Nc: int) -> np.ndarray:
     nrm = eL @ (~nanm).astype(np.float64) @ eR
     nrm[nanm] = np.nan
     return (eL @ A @ eR) / nrm
+
+
+# --------------------------------------------------------------------------
+# Step length and line search of one SWD iteration (SWD.m:203-322)
+# --------------------------------------------------------------------------
+
+def swd_step_model(vs, dk, alpha, f, vsmin, vsmax) -> np.ndarray:
+    """SWD.m:206-207: vs1 = vs + alpha*f*dk_vs, clipped to [vsmin, vsmax] (NaN passes)."""
+    vs1 = vs + (alpha * f) * dk
+    vs1[vs1 < vsmin] = vsmin
+    vs1[vs1 > vsmax] = vsmax
+    return vs1
+
+
+def swd_line_search(vs, dk, res0, misfit, pur, vsmin, vsmax, f1, f2=float("nan"), res2=float("nan"),
+                    alpha=None):
+    """SWD.m:203-322 for one iteration.  ``misfit(vs1)`` is the mean
+    right-side pick misfit of a forward-only batch of model vs1; ``dk`` is
+    the smoothed gradient; f1 / f2 / res2 are the script's carried values.
+    Returns (accepted model, dict of alpha, gama, res0, f1, f2, res1, res2,
+    res3, forwards)."""
+    if alpha is None:
+        alpha = np.sqrt(np.sum(vs * vs)) / np.sqrt(np.sum(dk * dk)) * pur          # SWD.m:203-205
+    n = 0
+
+    def trial(f):
+        nonlocal n
+        n += 1
+        return misfit(swd_step_model(vs, dk, alpha, f, vsmin, vsmax))
+
+    res1 = trial(f1)                                                               # SWD.m:206-229
+    if res1 > res0:
+        while res1 > res0 and f1 > 0.0001:                                         # SWD.m:233-256
+            f2 = f1
+            res2 = res1
+            f1 = f1 * 0.5
+            res1 = trial(f1)
+    else:
+        f2 = f1 * 2                                                                # SWD.m:257-278
+        res2 = trial(f2)
+    with np.errstate(divide="ignore", invalid="ignore"):  # IEEE division as MATLAB: 0/0 = NaN, x/0 = +-Inf
+        gama = float(np.float64(f1 ** 2 * (res0 - res2) + f2 ** 2 * (res1 - res0)) /
+                     np.float64(2 * res0 * (f1 - f2) + 2 * res1 * f2 - 2 * res2 * f1))
+    if np.isinf(gama):                                                             # SWD.m:279-284
+        gama = 0
+    res3 = trial(gama)                                                             # SWD.m:286-306
+    if res3 > res1 or res3 > res2:                                                 # SWD.m:308-322
+        if res1 > res2:
+            res0 = res2
+            gama = f2
+        else:
+            res0 = res1
+            gama = f1
+    else:
+        res0 = res3
+    vs1 = swd_step_model(vs, dk, alpha, gama, vsmin, vsmax)
+    return vs1, {"alpha": alpha, "gama": gama, "res0": res0, "f1": f1, "f2": f2, "res1": res1, "res2": res2,
+                 "res3": res3, "forwards": n}
 
 
 # --------------------------------------------------------------------------

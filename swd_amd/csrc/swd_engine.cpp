@@ -110,17 +110,27 @@ int Engine::create(const swd_problem* prob, hipStream_t stream)
         (rc = traces.alloc(S * ntr * sizeof(float))) ||
         (rc = snaps.alloc(S * (size_t)nrec * 3 * splane * sizeof(float))) ||
         (rc = adjsrc.alloc(S * ntr * sizeof(float))) || (rc = imgs.alloc(S * 3 * splane * sizeof(double))) ||
-        (rc = outs.alloc(4 * splane * sizeof(double))))
+        (rc = outs.alloc(4 * splane * sizeof(double))) || (rc = vp_d.alloc(splane * sizeof(double))) ||
+        (rc = geo.alloc((size_t)nzbc * nxbc * sizeof(double))) || (rc = vs_cur.alloc(splane * sizeof(double))) ||
+        (rc = vs_tr.alloc(splane * sizeof(double))) || (rc = dk_sm.alloc(splane * sizeof(double))) ||
+        (rc = stats.alloc(4 * sizeof(double))))
         return rc;
     SWD_HIP(hipMemsetAsync(coefs.p, 0, coefs.bytes, st));
+    {
+        std::vector<double> g((size_t)nzbc * nxbc);
+        damp_geometry(p.nz, p.nx, p.nbc, p.dx, pt_m, g.data());
+        SWD_HIP(hipMemcpyAsync(geo.p, g.data(), geo.bytes, hipMemcpyHostToDevice, st));
+        SWD_HIP(hipStreamSynchronize(st));
+    }
     return kOk;
 }
 
 size_t Engine::footprint_bytes() const
 {
-    const DevBuf* b[] = {&coefs, &vs_d, &wavelet, &state, &strain, &traces, &snaps, &adjsrc, &imgs, &outs,
+    const DevBuf* b[] = {&coefs, &vs_d, &wavelet, &state, &strain, &traces, &snaps, &adjsrc, &imgs, &outs, &djobs, &dbmap,
                          &disp.W, &disp.D, &disp.X, &disp.MM, &disp.ml, &disp.mlr, &disp.GMM, &disp.GD,
-                         &disp.Bm, &disp.crpre, &disp.cr0, &disp.res, &disp.status};
+                         &disp.Bm, &disp.crpre, &disp.cr0, &disp.res, &disp.status, &vp_d, &geo, &vs_cur,
+                         &vs_tr, &dk_sm, &stats, &disp.dcols};
     size_t t = 0;
     for (auto* x : b) t += x->bytes;
     return t;
@@ -153,8 +163,123 @@ int Engine::set_model(const double* vp, const double* vs)
     }
     SWD_HIP(hipMemcpyAsync(coefs.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, st));
     SWD_HIP(hipMemcpyAsync(vs_d.p, vs, splane * sizeof(double), hipMemcpyHostToDevice, st));
+    SWD_HIP(hipMemcpyAsync(vp_d.p, vp, splane * sizeof(double), hipMemcpyHostToDevice, st));
     SWD_HIP(hipStreamSynchronize(st));
     have_model = true;
+    return kOk;
+}
+
+int Engine::set_vs_device(const double* vs_dev)
+{
+    if (!have_model) { set_error("set_model must run once first (it fixes vp)"); return kErrArg; }
+    int rc;
+    double h[4];
+    if ((rc = launch_model_stats(st, vp_d.as<double>(), vs_dev, splane, stats.as<double>()))) return rc;
+    SWD_HIP(hipMemcpyAsync(h, stats.p, sizeof h, hipMemcpyDeviceToHost, st));
+    SWD_HIP(hipStreamSynchronize(st));
+    const double vmin = h[0], vmax = h[1];
+    if (!(vmax * p.dt / p.dx <= 1.0 / std::sqrt(2.0))) {
+        set_error("CFL violated: vmax*dt/dx = %.4f > 0.7071", vmax * p.dt / p.dx);
+        return kErrCfl;
+    }
+    vpmax = vmax;
+    const double kappa = damp_kappa(vmin, p.nbc, p.dx);
+    if ((rc = launch_prep_coefs(st, vp_d.as<double>(), vs_dev, geo.as<double>(), kappa, p.nz, p.nx, p.nbc, pt_m, nzbc,
+                                nxbc, pitch, plane, p.dt, p.dt / p.dx, coefs.as<float>())))
+        return rc;
+    if (vs_dev != vs_d.as<double>())
+        SWD_HIP(hipMemcpyAsync(vs_d.p, vs_dev, splane * sizeof(double), hipMemcpyDeviceToDevice, st));
+    SWD_HIP(hipStreamSynchronize(st));
+    return kOk;
+}
+
+// Mean pick misfit of the right side (SWD.m:195 and the line search use
+// mean(res_r)) over the shots whose right branch runs.
+double Engine::mean_right(const double* res_r) const
+{
+    double s = 0.0;
+    int n = 0;
+    for (int k = 0; k < p.nshots; ++k)
+        if (disp.cols[k].enabled) { s += res_r[k]; ++n; }
+    return n ? s / n : 0.0;
+}
+
+int Engine::iterate(swd_iteration* it)
+{
+    if (!it) { set_error("null iteration record"); return kErrArg; }
+    if (!have_model || !have_wavelet) { set_error("model and wavelet must be set first"); return kErrArg; }
+    int rc;
+    const size_t S = p.nshots;
+    std::vector<double> rr(S), rl(S);
+    // gradient of the current model (SWD.m:171-200) and its misfit
+    if ((rc = gradient(nullptr, rr.data(), rl.data(), nullptr, nullptr))) return rc;
+    double res0 = mean_right(rr.data());
+    it->residual = res0;
+    SWD_HIP(hipMemcpyAsync(vs_cur.p, vs_d.p, splane * sizeof(double), hipMemcpyDeviceToDevice, st));
+    // dk_vs = smooth2a(dk_vs, smoothZ, smoothX) (SWD.m:201)
+    if ((rc = launch_smooth2a(st, outs.as<double>(), p.nz, p.nx, it->smooth_z, it->smooth_x, dk_sm.as<double>())))
+        return rc;
+    // alpha = ||vs|| / ||dk|| * pur (SWD.m:203-205)
+    double h[4];
+    if ((rc = launch_model_stats(st, vs_cur.as<double>(), dk_sm.as<double>(), splane, stats.as<double>()))) return rc;
+    SWD_HIP(hipMemcpyAsync(h, stats.p, sizeof h, hipMemcpyDeviceToHost, st));
+    SWD_HIP(hipStreamSynchronize(st));
+    const double alpha = std::sqrt(h[2]) / std::sqrt(h[3]) * it->pur;
+    it->alpha = alpha;
+    it->forwards = 0;
+    // vs1 = clip(vs + alpha * f * dk) (SWD.m:206-207) and its forward-only misfit (SWD.m:215-229)
+    auto step = [&](double f) -> int {
+        return launch_model_step(st, vs_cur.as<double>(), dk_sm.as<double>(), alpha * f, it->vsmin, it->vsmax,
+                                 vs_tr.as<double>(), splane);
+    };
+    auto trial = [&](double f, double* res) -> int {
+        int r;
+        if ((r = step(f)) || (r = set_vs_device(vs_tr.as<double>())) ||
+            (r = forward_picks(rr.data(), rl.data(), nullptr, nullptr)))
+            return r;
+        ++it->forwards;
+        *res = mean_right(rr.data());
+        return kOk;
+    };
+    double f1 = it->f1, f2 = it->f2, res1 = 0, res2 = it->res2, res3 = 0;
+    if ((rc = trial(f1, &res1))) return rc;
+    if (res1 > res0) {  // SWD.m:233-256: halve while the misfit grows
+        while (res1 > res0 && f1 > 0.0001) {
+            f2 = f1;
+            res2 = res1;
+            f1 = f1 * 0.5;
+            if ((rc = trial(f1, &res1))) return rc;
+        }
+    } else {  // SWD.m:257-278: double once
+        f2 = f1 * 2;
+        if ((rc = trial(f2, &res2))) return rc;
+    }
+    // parabola through (0, res0), (f1, res1), (f2, res2) (SWD.m:279-287)
+    double gama = (f1 * f1 * (res0 - res2) + f2 * f2 * (res1 - res0)) /
+                  (2 * res0 * (f1 - f2) + 2 * res1 * f2 - 2 * res2 * f1);
+    if (std::isinf(gama)) gama = 0;
+    if ((rc = trial(gama, &res3))) return rc;
+    // keep the better trial when the parabola step is worse (SWD.m:308-322)
+    if (res3 > res1 || res3 > res2) {
+        if (res1 > res2) {
+            res0 = res2;
+            gama = f2;
+        } else {
+            res0 = res1;
+            gama = f1;
+        }
+        if ((rc = step(gama)) || (rc = set_vs_device(vs_tr.as<double>()))) return rc;
+    } else {
+        res0 = res3;
+    }
+    // vs = vs1: the accepted model stays in vs_d with its coefficient planes
+    it->res0 = res0;
+    it->gama = gama;
+    it->f1 = f1;
+    it->f2 = f2;
+    it->res1 = res1;
+    it->res2 = res2;
+    it->res3 = res3;
     return kOk;
 }
 
@@ -423,6 +548,32 @@ int swd_engine_forward_picks(swd_engine* e, double* res_r, double* res_l, double
     return e ? e->e.forward_picks(res_r, res_l, cr_r, cr_l) : kErrArg;
 }
 
+int swd_engine_iterate(swd_engine* e, swd_iteration* it) { return e ? e->e.iterate(it) : kErrArg; }
+
+int swd_smooth2a(const double* in, int nz, int nx, int nr, int nc, double* out)
+{
+    if (!in || !out || nz < 1 || nx < 1 || nr < 0 || nc < 0) { set_error("bad smooth2a arguments"); return kErrArg; }
+    Stream s;
+    DevBuf a, b;
+    int rc;
+    const size_t n = (size_t)nz * nx * sizeof(double);
+    if ((rc = s.create()) || (rc = a.alloc(n)) || (rc = b.alloc(n))) return rc;
+    SWD_HIP(hipMemcpyAsync(a.p, in, n, hipMemcpyHostToDevice, s.s));
+    if ((rc = launch_smooth2a(s.s, a.as<double>(), nz, nx, nr, nc, b.as<double>()))) return rc;
+    SWD_HIP(hipMemcpyAsync(out, b.p, n, hipMemcpyDeviceToHost, s.s));
+    SWD_HIP(hipStreamSynchronize(s.s));
+    return kOk;
+}
+
+int swd_engine_read_model(swd_engine* e, double* vs)
+{
+    if (!e || !vs) { set_error("null argument"); return kErrArg; }
+    Engine& g = e->e;
+    SWD_HIP(hipMemcpyAsync(vs, g.vs_d.p, g.splane * sizeof(double), hipMemcpyDeviceToHost, g.st));
+    SWD_HIP(hipStreamSynchronize(g.st));
+    return kOk;
+}
+
 int swd_engine_set_fd_path(swd_engine* e, int mode)
 {
     if (!e) return kErrArg;
@@ -555,6 +706,7 @@ int swd_engine_read(swd_engine* e, int what, void* dst, long long bytes)
     case 2: src = &g.outs; n = 4 * g.splane * sizeof(double); break;          // dk_vs, g_cl, g_cm, g_il
     case 3: src = &g.disp.ml; n = g.disp.ml.bytes; break;                     // [col][npair][np] float
     case 4: src = &g.imgs; n = g.imgs.bytes; break;                           // [shot][3][nz][nx] double
+    case 5: src = &g.dk_sm; n = g.dk_sm.bytes; break;                         // smoothed dk_vs, nz x nx double
     default: set_error("unknown buffer %d", what); return kErrArg;
     }
     if ((size_t)bytes < n) { set_error("destination too small (%lld < %zu)", bytes, n); return kErrArg; }

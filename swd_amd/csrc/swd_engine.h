@@ -55,6 +55,12 @@ public:
     DevBuf imgs;      // [shot][3][nz][nx] double
     DevBuf outs;      // dk_vs, g_cl, g_cm, g_il (nz x nx double each)
     DevBuf djobs, dbmap;
+    DevBuf vp_d;      // nz x nx double, the current vp (column-major)
+    DevBuf geo;       // nzbc x nxbc double, damping geometry factors (column-major)
+    DevBuf vs_cur;    // nz x nx double, model at the start of an iteration
+    DevBuf vs_tr;     // nz x nx double, trial model of the line search
+    DevBuf dk_sm;     // nz x nx double, smoothed gradient
+    DevBuf stats;     // 4 doubles: min, max, sum of squares (model_stats)
     DispPlan disp;
     StageTimes times;
     int timing_samples = 8;
@@ -64,6 +70,12 @@ public:
     int set_model(const double* vp, const double* vs);
     int set_wavelet(const double* s);
     int set_observed_picks(const double* cr0_r, const double* cr0_l);
+    // rebuild the coefficient planes for a device-resident vs (vp unchanged)
+    int set_vs_device(const double* vs_dev);
+    // one SWD iteration (SWD.m:171-335): gradient, smoothing, line search,
+    // model update, all on the device
+    int iterate(swd_iteration* it);
+    double mean_right(const double* res_r) const;
     size_t footprint_bytes() const;
 
     // building blocks (all enqueue on st; nothing synchronises unless stated)

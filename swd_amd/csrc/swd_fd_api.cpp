@@ -362,6 +362,43 @@ static void pad_rep(const double* p0, int nz, int nx, int nbc, int top, double* 
     }
 }
 
+// damp_circle (core/damp_circle.m:8-67) with a free surface, kappa factored
+// out: side ramps (r/a)^2 on the model rows, bottom ramp, lower corners
+// (dist/nbc)^2, top rows 0.  g is nzbc x nxbc column-major.
+void damp_geometry(int nz, int nx, int nbc, double dx, int top, double* g)
+{
+    const int nzbc = nz + top + nbc, nxbc = nx + 2 * nbc;
+    std::fill(g, g + (size_t)nzbc * nxbc, 0.0);
+    const double a = (nbc - 1) * dx;
+    std::vector<double> d1(nbc);
+    for (int k = 0; k < nbc; ++k) { const double r = k * dx / a; d1[k] = r * r; }
+    auto G = [&](int iz, int ix) -> double& { return g[(size_t)(ix - 1) * nzbc + (iz - 1)]; };
+    for (int iz = 1 + top; iz <= nz + top; ++iz)
+        for (int k = 1; k <= nbc; ++k) { G(iz, k) = d1[nbc - k]; G(iz, nx + nbc + k) = d1[k - 1]; }
+    for (int ix = 1 + nbc; ix <= nx + nbc; ++ix)
+        for (int k = 1; k <= nbc; ++k) G(nzbc - nbc + k, ix) = d1[k - 1];
+    for (int iz = nz + top + 1; iz <= nzbc; ++iz) {
+        const double dz = iz - nz - top;
+        for (int ix = 1; ix <= nbc; ++ix) {
+            const double dxl = ix - nbc - 1;
+            const double dist = std::sqrt(dxl * dxl + dz * dz);
+            G(iz, ix) = (dist / nbc) * (dist / nbc);
+        }
+        for (int ix = nx + nbc + 1; ix <= nxbc; ++ix) {
+            const double dxr = ix - nbc - nx;
+            const double dist = std::sqrt(dxr * dxr + dz * dz);
+            G(iz, ix) = (dist / nbc) * (dist / nbc);
+        }
+    }
+}
+
+// kappa = 3 vmin ln(1e7) / (2 a), a = (nbc - 1) dx (core/damp_circle.m)
+double damp_kappa(double vmin, int nbc, double dx)
+{
+    const double a = (nbc - 1) * dx;
+    return 3.0 * vmin * std::log(10000000.0) / (2.0 * a);
+}
+
 int prepare_model(const double* vp, const double* vs, int nz, int nx, int nbc, double dx, double dt, int fd_order,
                   double* temp, double* ca, double* cl, double* cm, double* cm1, double* b, double* b1)
 {
@@ -382,31 +419,16 @@ int prepare_model(const double* vp, const double* vs, int nz, int nx, int nbc, d
     pad_rep(cm0.data(), nz, nx, nbc, top, cm);
     pad_rep(cl0.data(), nz, nx, nbc, top, cl);
     pad_rep(ca0.data(), nz, nx, nbc, top, ca);
-    // damp_circle with isfs = 1: side ramps, bottom ramp, lower corners; top rows 0
-    const double a = (nbc - 1) * dx;
-    const double kappa = 3.0 * vmin * std::log(10000000.0) / (2.0 * a);
-    std::vector<double> d1(nbc), damp(n, 0.0);
-    for (int k = 0; k < nbc; ++k) { const double r = k * dx / a; d1[k] = kappa * (r * r); }
-    auto D = [&](int iz, int ix) -> double& { return damp[(size_t)(ix - 1) * nzbc + (iz - 1)]; };
-    for (int iz = 1 + top; iz <= nz + top; ++iz)
-        for (int k = 1; k <= nbc; ++k) { D(iz, k) = d1[nbc - k]; D(iz, nx + nbc + k) = d1[k - 1]; }
-    for (int ix = 1 + nbc; ix <= nx + nbc; ++ix)
-        for (int k = 1; k <= nbc; ++k) D(nzbc - nbc + k, ix) = d1[k - 1];
-    for (int iz = nz + top + 1; iz <= nzbc; ++iz) {
-        const double dz = iz - nz - top;
-        for (int ix = 1; ix <= nbc; ++ix) {
-            const double dxl = ix - nbc - 1;
-            const double dist = std::sqrt(dxl * dxl + dz * dz);
-            D(iz, ix) = kappa * ((dist / nbc) * (dist / nbc));
-        }
-        for (int ix = nx + nbc + 1; ix <= nxbc; ++ix) {
-            const double dxr = ix - nbc - nx;
-            const double dist = std::sqrt(dxr * dxr + dz * dz);
-            D(iz, ix) = kappa * ((dist / nbc) * (dist / nbc));
-        }
-    }
+    // damp_circle with isfs = 1: damp = kappa(vmin) * geometry factor
+    std::vector<double> geo(n);
+    damp_geometry(nz, nx, nbc, dx, top, geo.data());
+    const double kappa = damp_kappa(vmin, nbc, dx);
     const double dtx = dt / dx;
-    for (size_t k = 0; k < n; ++k) { temp[k] = 1 - damp[k] * dt; b[k] = dtx; }
+    for (size_t k = 0; k < n; ++k) {
+        const double damp = kappa * geo[k];
+        temp[k] = 1 - damp * dt;
+        b[k] = dtx;
+    }
     const int fr = top - 1;                       // free-surface row (0-based)
     for (int j = 0; j < nxbc; ++j) {
         const size_t o = (size_t)j * nzbc + fr;

@@ -26,6 +26,17 @@ void zero_ring(T* plane, int nzbc, int nxbc, int pitch);
 int prepare_model(const double* vp, const double* vs, int nz, int nx, int nbc, double dx, double dt, int fd_order,
                   double* temp, double* ca, double* cl, double* cm, double* cm1, double* b, double* b1);
 int ricker(double f, double dt, int nt, double* w);
+void damp_geometry(int nz, int nx, int nbc, double dx, int top, double* g);
+double damp_kappa(double vmin, int nbc, double dx);
+
+// model update kernels (model_kernels.hip)
+int launch_prep_coefs(hipStream_t st, const double* vp, const double* vs, const double* geo, double kappa, int nz,
+                      int nx, int nbc, int top, int nzbc, int nxbc, int pitch, size_t plane, double dt, double dtx,
+                      float* coefs);
+int launch_smooth2a(hipStream_t st, const double* in, int nz, int nx, int Nr, int Nc, double* out);
+int launch_model_step(hipStream_t st, const double* vs, const double* dk, double step, double vsmin, double vsmax,
+                      double* out, size_t n);
+int launch_model_stats(hipStream_t st, const double* a, const double* b, size_t n, double* out);
 
 // First-fit sub-allocator over one allocation made once (device memory via
 // hipMalloc or pinned host memory via hipHostMalloc), with coalescing frees.

@@ -22,6 +22,13 @@ class Problem(ctypes.Structure):
                 ("is_index", _P), ("ns", _I)]
 
 
+class Iteration(ctypes.Structure):
+    """swd_iteration of include/swd_abi.h: parameters, carried state and results of one SWD iteration."""
+    _fields_ = [("pur", _D), ("vsmin", _D), ("vsmax", _D), ("smooth_z", _I), ("smooth_x", _I), ("f1", _D),
+                ("f2", _D), ("res2", _D), ("residual", _D), ("res0", _D), ("alpha", _D), ("gama", _D),
+                ("res1", _D), ("res3", _D), ("forwards", _I)]
+
+
 PROBLEM_KEYS = ("nz", "nx", "nbc", "fd_order", "nt", "in_wf", "dx", "dt", "ng", "gx0", "dg", "gz", "df", "fmin",
                 "fmax", "vmin", "vmax", "ds", "offset", "m", "M", "ini", "w", "beta", "ns")
 
@@ -120,6 +127,25 @@ class Engine:
 
     STAGES = {"forward": 1, "dispersion": 2, "adjoint_source": 3, "adjoint": 4, "assemble": 5}
 
+    def iterate(self, vsmin: float, vsmax: float, pur: float = -0.05, smooth_z: int = 1, smooth_x: int = 2,
+                state: dict | None = None) -> dict:
+        """One SWD iteration on the device (swd_engine_iterate): gradient,
+        smooth2a, line search and model update.  ``state`` carries f1 / f2 /
+        res2 between iterations (f1 = 0.5 at the first); the returned dict
+        holds the results and the new state."""
+        st = state or {}
+        it = Iteration(pur=pur, vsmin=vsmin, vsmax=vsmax, smooth_z=smooth_z, smooth_x=smooth_x,
+                       f1=st.get("f1", 0.5), f2=st.get("f2", float("nan")), res2=st.get("res2", float("nan")))
+        check(lib().swd_engine_iterate(self.h, ctypes.byref(it)))
+        return {name: getattr(it, name) for name, _ in Iteration._fields_}
+
+    def model(self) -> np.ndarray:
+        """The engine's current vs (nz x nx)."""
+        q = self.problem
+        vs = np.zeros((q["nz"], q["nx"]), order="F")
+        check(lib().swd_engine_read_model(self.h, ptr(vs)))
+        return vs
+
     def run(self, stage: str):
         """One stage of gradient() on the resident buffers."""
         check(lib().swd_engine_run(self.h, self.STAGES[stage]))
@@ -159,12 +185,14 @@ class Engine:
         info = self.info()
         shapes = {"gathers": (0, np.float32, (S, nt, ng)), "adjoint_sources": (1, np.float32, (S, nt, ng)),
                   "gradient": (2, np.float64, (4, nx, nz)), "spectra": (3, np.float32, (2 * S, self.npair, info["np"])),
-                  "images": (4, np.float64, (S, 3, nz, nx))}
+                  "images": (4, np.float64, (S, 3, nz, nx)), "smoothed_gradient": (5, np.float64, (nx, nz))}
         code, dt, shape = shapes[what]
         out = np.zeros(shape, dtype=dt)
         check(lib().swd_engine_read(self.h, code, ptr(out), out.nbytes))
         if what == "gradient":
             return np.transpose(out, (0, 2, 1))     # -> [4][nz][nx]
+        if what == "smoothed_gradient":
+            return out.T                             # column-major nz x nx
         return out
 
 
