@@ -170,6 +170,24 @@ def uneven_leg(steps=2):
                     "batch gathers of shot 1 of each model vs the FP64 GPU MEX path"}
 
 
+def iteration_leg(eng, mod, prob):
+    """One full SWD iteration of the production-shots workload on the device
+    (swd_engine_iterate: gradient, smooth2a, line search of forward-only
+    batches, model update), after the timed gradients; the engine's model
+    changes, so it runs last on this engine."""
+    vsmin, vsmax = float(np.min(mod["vs_true"])), float(np.max(mod["vs_true"]))
+    t0 = time.perf_counter()
+    r = eng.iterate(vsmin, vsmax, pur=-0.05, smooth_z=1, smooth_x=2, state={"f1": 0.5})
+    ms = 1000.0 * (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    eng.forward_picks()
+    fwd_ms = 1000.0 * (time.perf_counter() - t0)
+    return {"ms": round(ms, 1), "forward_batches": r["forwards"], "forward_batch_ms": round(fwd_ms, 1),
+            "residual": r["residual"], "res0": r["res0"], "gama": r["gama"], "alpha": r["alpha"],
+            "what": "SWD.m:171-335 for one k on 60 shots: gradient + smooth2a + line search (forward FD without "
+                    "snapshots, Radon, picks per trial) + model update, device-resident"}
+
+
 PEAK_MFMA_FP32_TFLOPS = 157.3  # MI355X dense FP32 MFMA
 
 
@@ -276,6 +294,7 @@ def main():
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven multi-model batch leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--no-dispersion", action="store_true", help="skip the dispersion-path leg")
+    ap.add_argument("--no-iteration", action="store_true", help="skip the device-resident inversion iteration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the shot-parallel multi-GPU "
@@ -378,6 +397,9 @@ def main():
                              "what": f"unfused per-statement kernels, one shot per launch, {args.plain_shots} shots",
                              "speedup": round(value / plain, 2)}
         e2.close()
+    if not args.no_iteration:
+        out["iteration"] = iteration_leg(eng, mod, prob)
+        progress(f"iteration: {out['iteration']['ms']} ms")
     eng.close()
     progress(f"headline: {value:.1f} Gcell-steps/s")
     if not args.no_dispersion:
