@@ -212,7 +212,19 @@ def dispersion_leg(betas=(5.0, 10.0, 20.0), reps=3):
     for b in betas:
         D.adjoint(b, want=False)
         adj_ms[f"{b:g}"] = round(min(D.adjoint(b, want=False)[1] for _ in range(reps)), 3)
+    # the BF16 split-precision variant of the DFT / inverse-DFT GEMMs
+    from swd_amd.dispersion import set_radon_precision
+    old = set_radon_precision(1)
+    try:
+        D.pick()
+        pick16 = min(D.pick()["ms"] for _ in range(reps))
+        r16 = D.pick()
+        D.adjoint(10.0, want=False)
+        adj16 = min(D.adjoint(10.0, want=False)[1] for _ in range(reps))
+    finally:
+        set_radon_precision(old)
     D.close()
+    same_bin = float(np.mean(np.abs(r16["cr_r"] - r["cr_r"]) <= 1.0))
     nt, ng, npair, S = prob["nt"], prob["ng"], D.npair, 60
     dft = 2.0 * (2 * npair) * min(nt, D.ccn) * ng * S
     slant = 8.0 * D.np * npair * D.sum_l
@@ -231,6 +243,12 @@ def dispersion_leg(betas=(5.0, 10.0, 20.0), reps=3):
                      "forward_tflops": round((dft + slant) / (pick_ms * 1e-3) / 1e12, 2),
                      "adjoint_tflops": round((dft + slant) / (amean * 1e-3) / 1e12, 2),
                      "peak_fp32_tflops": PEAK_MFMA_FP32_TFLOPS},
+            "bf16x3": {"pick_ms": round(pick16, 3), "adjoint_ms_beta10": round(adj16, 3),
+                       "forward_tflops": round((dft + slant) / (pick16 * 1e-3) / 1e12, 2),
+                       "adjoint_tflops": round((dft + slant) / (adj16 * 1e-3) / 1e12, 2),
+                       "right_picks_within_1_bin_of_fp32": round(same_bin, 4),
+                       "what": "DFT and inverse-DFT GEMMs in BF16 split precision (bf16 hi + lo, 3 MFMA products); "
+                               "slant stacks stay FP32"},
             "picks_within_1_bin_of_c_15_70hz": round(float(np.mean(np.concatenate(ok))), 4),
             "pick_columns_judged": len(ok),
             "what": "times include the small kernels of each stage (gather, |mm|, picks, softargmax, placement)"}

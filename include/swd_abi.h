@@ -294,6 +294,13 @@ int swd_dispersion_set_observed_picks(swd_dispersion *d, const double *cr0_r, co
 int swd_dispersion_pick(swd_dispersion *d, double *cr_r, double *cr_l, double *res_r, double *res_l, double *ms);
 int swd_dispersion_adjoint(swd_dispersion *d, double beta, float *adj, double *ms);
 
+/* Arithmetic of the Radon DFT and inverse-DFT GEMMs (engine, dispersion
+ * batch and RT* entry points): 0 = FP32 MFMA (default); 1 = BF16 split
+ * precision, each FP32 operand split into bf16 hi + lo and three bf16 MFMA
+ * products accumulated in FP32 (about 1e-5 relative).  Process-wide; returns
+ * the previous mode.  No reference counterpart. */
+int swd_set_radon_precision(int mode);
+
 /* Number of Radon frequency bins (rows of the pick arrays) of a problem. */
 int swd_problem_npair(const swd_problem *p, int *npair);
 

@@ -68,6 +68,7 @@ SIGNATURES = {
     "swd_engine_read": (_I, [_P, _I, _P, ctypes.c_longlong]),
     "swd_batch_gradient": (_I, [_P, _I, _P, _P]),
     "swd_problem_npair": (_I, [_P, _P]),
+    "swd_set_radon_precision": (_I, [_I]),
     "swd_engine_iterate": (_I, [_P, _P]),
     "swd_engine_read_model": (_I, [_P, _P]),
     "swd_smooth2a": (_I, [_P, _I, _I, _I, _I, _P]),

@@ -102,6 +102,93 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(GemmArgs g)
 }
 
 // ---------------------------------------------------------------------------
+// The same GEMM in BF16 split precision ("BF16 x 3") on the bf16 MFMA units:
+// each FP32 operand is split into x = hi + lo with hi = bf16(x) and
+// lo = bf16(x - hi) (round to nearest even), and a b = hi_a hi_b + hi_a lo_b
+// + lo_a hi_b accumulates in FP32 (the lo_a lo_b term, ~2^-16 relative, is
+// dropped).  The split happens once per element on the way into LDS, where
+// both planes are stored k-contiguous so that one 8-byte read gives a lane
+// its 4 k-values of v_mfma_f32_16x16x16_bf16 (A[m = l%16][k = 4(l/16)+i],
+// B[k = 4(l/16)+i][n = l%16], D[m = 4(l/16)+r][n = l%16]).
+// ---------------------------------------------------------------------------
+typedef short v4s __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short bf16_rn(float x)
+{
+    const unsigned u = __float_as_uint(x);
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+constexpr int GKP = GBK + 4;  // LDS row of 16 k-values (+4 to spread banks)
+
+__global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GemmArgs g)
+{
+    __shared__ unsigned short Ah[GBM][GKP], Al[GBM][GKP], Bh[GBN][GKP], Bl[GBN][GKP];
+    const int b = blockIdx.z;
+    const int m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const float* A = g.A + (size_t)b * g.sAb;
+    const float* B = g.B + (size_t)b * g.sBb;
+    const bool a_mfast = g.sAm == 1;
+    const bool b_nfast = g.sBn == 1;
+    const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+    v4f acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    const int kq = 4 * (lane >> 4), lr = lane & 15;
+    for (int k0 = 0; k0 < g.K; k0 += GBK) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + 256 * q;
+            int mm, kk;
+            if (a_mfast) { mm = e % GBM; kk = e / GBM; } else { kk = e % GBK; mm = e / GBK; }
+            const int gm = m0 + mm, gk = k0 + kk;
+            const float av = (gm < g.M && gk < g.K) ? A[(size_t)gm * g.sAm + (size_t)gk * g.sAk] : 0.f;
+            const unsigned short ah = bf16_rn(av);
+            Ah[mm][kk] = ah;
+            Al[mm][kk] = bf16_rn(av - bf16_to_f(ah));
+            int nn;
+            if (b_nfast) { nn = e % GBN; kk = e / GBN; } else { kk = e % GBK; nn = e / GBK; }
+            const int gn = n0 + nn, gk2 = k0 + kk;
+            const float bv = (gn < g.N && gk2 < g.K) ? B[(size_t)gk2 * g.sBk + (size_t)gn * g.sBn] : 0.f;
+            const unsigned short bh = bf16_rn(bv);
+            Bh[nn][kk] = bh;
+            Bl[nn][kk] = bf16_rn(bv - bf16_to_f(bh));
+        }
+        __syncthreads();
+        v4s ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            ah[i] = *(const v4s*)&Ah[wm + i * 16 + lr][kq];
+            al[i] = *(const v4s*)&Al[wm + i * 16 + lr][kq];
+            bh[i] = *(const v4s*)&Bh[wn + i * 16 + lr][kq];
+            bl[i] = *(const v4s*)&Bl[wn + i * 16 + lr][kq];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[i], bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bh[j], acc[i][j], 0, 0, 0);
+            }
+        __syncthreads();
+    }
+    float* C = g.C + (size_t)b * g.sCb;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gm = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+                const int gn = n0 + wn + j * 16 + (lane & 15);
+                if (gm < g.M && gn < g.N) C[(size_t)gm * g.sCm + (size_t)gn * g.sCn] = g.alpha * acc[i][j][r];
+            }
+}
+
+// ---------------------------------------------------------------------------
 // Window gather: X[k][l][col] = D(shot, k, trace(col, l)), 0 beyond L(col).
 // D layout: D[shot][k2][g] with Re at k2 = k and Im at k2 = npair + k.
 // ---------------------------------------------------------------------------
@@ -519,7 +606,8 @@ int launch_gemm_f32(hipStream_t st, const GemmArgs& g, int batch)
 {
     if (g.M <= 0 || g.N <= 0 || batch <= 0) return kOk;
     const dim3 grid(div_up(g.N, GBN), div_up(g.M, GBM), batch);
-    gemm_f32_kernel<<<grid, 256, 0, st>>>(g);
+    if (radon_precision() == 1) gemm_bf16x3_kernel<<<grid, 256, 0, st>>>(g);
+    else gemm_f32_kernel<<<grid, 256, 0, st>>>(g);
     SWD_HIP(hipGetLastError());
     return kOk;
 }

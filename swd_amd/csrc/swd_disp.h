@@ -38,6 +38,11 @@ struct SlantArgs {
 
 int launch_dft_table(hipStream_t st, float* W, int nt_eff, int npair, int c0, int ccn);
 int launch_gemm_f32(hipStream_t st, const GemmArgs& g, int batch);
+
+// Arithmetic of the Radon DFT / inverse-DFT GEMMs: 0 = FP32 MFMA (default),
+// 1 = BF16 split precision (three bf16 MFMA products per FP32 product).
+int radon_precision();
+int set_radon_precision(int mode);
 int launch_radon_gather(hipStream_t st, const float* D, const DispCol* cols, int ncol, int npair, int ng, int Lmax,
                         float2* X);
 int launch_slant_stack(hipStream_t st, const SlantArgs& a, bool adjoint);

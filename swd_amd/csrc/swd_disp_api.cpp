@@ -3,6 +3,7 @@
 // LHDispPick and ADWDgrad_1 of the C ABI.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -13,6 +14,10 @@
 #include "swd_disp_plan.h"
 
 namespace swd {
+
+static std::atomic<int> g_radon_prec{0};
+int radon_precision() { return g_radon_prec.load(); }
+int set_radon_precision(int mode) { return g_radon_prec.exchange(mode == 1 ? 1 : 0); }
 
 // MATLAB round (half away from zero) and fix(1./df./dt) (RTr.m:16,19-20)
 static int mround(double x) { return (int)(x >= 0 ? std::floor(x + 0.5) : -std::floor(-x + 0.5)); }
@@ -296,6 +301,13 @@ static int adwdgrad1_impl(int nt, int ng, int ns, int npair, int is, double w, i
 using namespace swd;
 
 extern "C" {
+
+int swd_set_radon_precision(int mode)
+{
+    if (mode != 0 && mode != 1) { set_error("Radon precision must be 0 (FP32) or 1 (BF16 x 3)"); return kErrArg; }
+    return set_radon_precision(mode);
+}
+
 
 int swd_radon(int side, const double* seismo, int nt, int ng, int is, double df, double dt, int np, double vmin,
               double vmax, double fmin, double fmax, double ds, int offset, int m, int M, float* ml, int* dataLen,

@@ -119,6 +119,15 @@ def ADWDgrad_1(nt, ng, ns, npair, is_, w, m, M, SoftArgNorm, grad_outputr, grad_
     return out
 
 
+def set_radon_precision(mode: int) -> int:
+    """Radon DFT / inverse-DFT GEMM arithmetic: 0 = FP32 MFMA, 1 = BF16 split
+    precision (bf16 hi + lo, three bf16 MFMA products).  Returns the old mode."""
+    rc = load().swd_set_radon_precision(int(mode))
+    if rc < 0:
+        check(rc)
+    return rc
+
+
 class DispersionBatch:
     """The dispersion path alone on caller-provided gathers, batched over
     every (shot, side) column (swd_dispersion_* in include/swd_abi.h): RTr /

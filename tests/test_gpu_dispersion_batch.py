@@ -80,3 +80,34 @@ def test_batch_adjoint_vs_oracle(chirp, beta):
         e = rel(adj[s], a_ref)
         print(f"beta {beta} shot {s + 1}: adjoint source rel-L2 vs FP64 {e:.3e} ({ms:.2f} ms for the batch)")
         assert e < 1e-4
+
+
+def test_bf16x3_radon_matches_fp32_and_oracle(chirp):
+    """The BF16 split-precision DFT / inverse-DFT GEMMs (swd_set_radon_precision(1)):
+    picks within one bin of the FP32 path, adjoint sources within 1e-4 of the oracle."""
+    from swd_amd.dispersion import set_radon_precision
+    prob, g, D, picks, ref = chirp
+    old = set_radon_precision(1)
+    try:
+        D.set_observed_picks(np.ones((D.npair, NSHOT)), np.ones((D.npair, NSHOT)))
+        p16 = D.pick()
+        for key, side in (("cr_r", "r"), ("cr_l", "l")):
+            for s in range(NSHOT):
+                if len(ref[s][side][2]["uxtpos"]) < 200:
+                    continue  # short windows have near-flat spectra: picks there move with any rounding
+                assert np.max(np.abs(p16[key][:, s] - picks[key][:, s])) <= 1.0
+        r = np.random.default_rng(3)
+        gr = np.round(r.normal(0, 5, (D.npair, NSHOT)))
+        gl = np.round(r.normal(0, 5, (D.npair, NSHOT)))
+        D.set_observed_picks(p16["cr_r"] - gr, p16["cr_l"] - gl)
+        adj, ms = D.adjoint(10.0)
+    finally:
+        set_radon_precision(old)
+    q = prob
+    for s in range(NSHOT):
+        # the oracle with the picks the bf16 path made (identical g by construction)
+        a_ref = O.ADWDgrad_1(q["nt"], q["ng"], NSHOT, D.npair, s + 1, q["w"], q["m"], q["M"], 10.0, gr[:, s],
+                             gl[:, s], ref[s]["r"][2], ref[s]["l"][2])
+        e = rel(adj[s], a_ref)
+        print(f"bf16x3 shot {s + 1}: adjoint source rel-L2 vs FP64 {e:.3e}")
+        assert e < 1e-4
