@@ -188,6 +188,49 @@ def iteration_leg(eng, mod, prob):
                     "snapshots, Radon, picks per trial) + model update, device-resident"}
 
 
+def field_leg():
+    """The outcrop survey at full size (fieldexamples/outcrop/outcrop.m: 40 x
+    240 cells at 0.25 m, 40-cell borders, 60 shots, 60 receivers, nt 7450,
+    npair 651 at df 0.1 Hz, 50-trace windows) with the outcrop variant
+    (norm_trace, strongest-column pick start, windowed misfit, peak-normalised
+    adjoint sources, vp = 1.732 vs): one gradient and one device iteration.
+    The observed picks are synthetic (a forward of a model with a +8% Vs
+    anomaly), standing in for the survey's manual picks."""
+    from swd_amd import field as F
+    from swd_amd.engine import Engine
+    prob, mod = F.outcrop(offset=50)
+    vs_true = mod["vs"] * (1 + 0.08 * np.exp(-((np.arange(240)[None, :] - 120) ** 2 +
+                                                (np.arange(40)[:, None] - 15.0) ** 2) / (2 * 12.0 ** 2)))
+    e = Engine(prob)
+    try:
+        e.set_variant("outcrop")
+        e.set_wavelet(mod["s"])
+        e.set_model(1.732 * vs_true, vs_true)
+        e.observe()
+        e.set_model(mod["vp"], mod["vs"])
+        tune = e.autotune()
+        e.gradient()
+        t0 = time.perf_counter()
+        e.gradient()
+        g_s = time.perf_counter() - t0
+        t = e.timings()
+        i = e.info()
+        cell_steps = e.nshots * i["nzbc"] * i["nxbc"] * (2 * prob["nt"] - 1)
+        vsmin, vsmax = F.OUTCROP_VS_BOUNDS
+        t0 = time.perf_counter()
+        r = e.iterate(vsmin, vsmax, pur=-0.05, smooth_z=1, smooth_x=2, state={"f1": 0.5})
+        it_ms = 1000.0 * (time.perf_counter() - t0)
+    finally:
+        e.close()
+    return {"value": round(cell_steps / g_s / 1e9, 3), "unit": "Gcell-steps/s", "ms_per_gradient": round(1000 * g_s, 2),
+            "stage_ms": {k: round(t[k], 2) for k in ("fwd_ms", "disp_ms", "adj_ms", "asm_ms")},
+            "fd_launch": {"fwd": list(tune["fwd"]), "adj": list(tune["adj"])},
+            "npair": i["npair"], "grid": [i["nzbc"], i["nxbc"]], "nt": prob["nt"], "shots": e.nshots,
+            "iteration_ms": round(it_ms, 1), "iteration_forward_batches": r["forwards"],
+            "residual": r["residual"], "res0": r["res0"],
+            "what": "outcrop.m survey, one SWD gradient and one device iteration with the field variant"}
+
+
 PEAK_MFMA_FP32_TFLOPS = 157.3  # MI355X dense FP32 MFMA
 
 
@@ -313,6 +356,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--no-dispersion", action="store_true", help="skip the dispersion-path leg")
     ap.add_argument("--no-iteration", action="store_true", help="skip the device-resident inversion iteration")
+    ap.add_argument("--no-field", action="store_true", help="skip the outcrop-survey leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the shot-parallel multi-GPU "
@@ -423,6 +467,9 @@ def main():
     if not args.no_dispersion:
         out["dispersion"] = dispersion_leg()
         progress(f"dispersion: picks {out['dispersion']['pick_ms']} ms for 60 shots")
+    if not args.no_field:
+        out["field_outcrop"] = field_leg()
+        progress(f"outcrop survey: {out['field_outcrop']['ms_per_gradient']} ms per gradient")
     if not args.no_uneven:
         out["uneven_batch"] = uneven_leg()
     if not args.no_serving:

@@ -192,6 +192,35 @@ int swd_engine_forward_picks(swd_engine *e, double *res_r, double *res_l, double
 /* 0 = fused batched kernels (default); 1 = the plain per-statement kernels,
  * one shot per launch (the unfused comparator).  Returns the previous mode. */
 int swd_engine_set_fd_path(swd_engine *e, int mode);
+/* Variants of the field-data scripts (fieldexamples/outcrop/outcrop.m and
+ * fieldexamples/QadimahFault/QademahSWD3.m with their core/ copies); all
+ * zero is SWD.m.
+ *   norm_trace     divide every gather trace by its peak |x| before the Radon
+ *                  of each dispersion pass (norm_trace on the predicted data,
+ *                  outcrop.m:164, QademahSWD3.m:164; core/norm_trace.m:8-16).
+ *                  No derivative is taken through it, as in the scripts; a
+ *                  zero trace gives NaN spectra (0/0 in norm_trace).
+ *   auto_start     pick from the frequency column holding the largest |mm|
+ *                  instead of the problem's ini ([~,ini] = max(max(mlr)),
+ *                  outcrop/core/LHDispPick.m:3-9, QadimahFault/core/LHDispPick.m:4-10),
+ *                  starting at that column's peak.
+ *   res_first, res_last  the pick misfit averages cr(res_first:end-res_last)
+ *                  (1-based; 15, 15 in outcrop/core/LHDispPick.m:21; 50, 30 in
+ *                  QadimahFault/core/LHDispPick.m:25); 0, 0 = every column.
+ *   adj_peak_norm  divide each shot's adjoint source by its peak |a|
+ *                  (a_data_res./max(abs(a_data_res(:))), ADWDgrad_1.m:79 of
+ *                  both field copies).
+ *   vp_ratio       > 0: every model change of swd_engine_iterate sets
+ *                  vp = vp_ratio * vs1 (vp = vs1*1.732, outcrop.m:217,245,271,
+ *                  301,331); 0 keeps vp fixed (SWD.m).  The dispersion batch
+ *                  ignores it.
+ * The variant applies to every later dispersion pass of the handle (observed
+ * picks included). */
+typedef struct swd_variant {
+    int norm_trace, auto_start, res_first, res_last, adj_peak_norm;
+    double vp_ratio;
+} swd_variant;
+int swd_engine_set_variant(swd_engine *e, const swd_variant *v);
 /* One stage of swd_engine_gradient on the resident buffers: 1 forward FD +
  * snapshots, 2 Radon + picks, 3 adjoint sources, 4 adjoint FD + imaging,
  * 5 reduction + dk_vs.  swd_engine_set_picks overwrites the picks between
@@ -291,6 +320,8 @@ int swd_dispersion_destroy(swd_dispersion *d);
 int swd_dispersion_info(swd_dispersion *d, long long *vals, int n);
 int swd_dispersion_set_gathers(swd_dispersion *d, const float *gathers);
 int swd_dispersion_set_observed_picks(swd_dispersion *d, const double *cr0_r, const double *cr0_l);
+/* The field-script variants of swd_engine_set_variant (vp_ratio ignored). */
+int swd_dispersion_set_variant(swd_dispersion *d, const swd_variant *v);
 int swd_dispersion_pick(swd_dispersion *d, double *cr_r, double *cr_l, double *res_r, double *res_l, double *ms);
 int swd_dispersion_adjoint(swd_dispersion *d, double beta, float *adj, double *ms);
 

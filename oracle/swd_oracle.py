@@ -49,6 +49,11 @@ PARITY = {
     "ADWDgrad_1": "finite differences of the soft-argmax misfit, slant-stack dot-product test, "
                   "one-sided real(ifft) closed form",
     "smooth2a": "box mean with truncated edges, brute force",
+    "norm_trace": "closed form: each column's peak |x| becomes 1 with signs kept, invariance under positive "
+                  "column gains, NaN samples skipped by the max, a zero column gives NaN",
+    "LHDispPick_field": "hand-built spectra whose largest |mm| sits in a known column (start column and row), "
+                        "closed-form windowed misfit, pickFDCValue == pickFDC1 when the given row is the argmax",
+    "ADWDgrad_1 peak_norm": "closed form: peak |a| becomes 1, the result is the unnormalised source over its peak",
     "swd_line_search": "quadratic misfit in the step factor: the parabola step lands on the exact minimum; "
                        "the halving branch, the doubling branch, the fallback to the better trial and the "
                        "clipping each checked on closed-form misfits",
@@ -368,8 +373,10 @@ def _nanmax_first(col) -> int:
     return int(np.nanargmax(col)) + 1
 
 
-def pickFDC1(ml, lp: int, fg: float = 1.5, lfg: Optional[float] = None) -> np.ndarray:
-    """core/pickFDC1.m:1-131; returns cr_0 (1 x b, float, 1-based velocity index)."""
+def pickFDC1(ml, lp: int, fg: float = 1.5, lfg: Optional[float] = None, lpvalue: Optional[int] = None) -> np.ndarray:
+    """core/pickFDC1.m:1-131; returns cr_0 (1 x b, float, 1-based velocity index).
+    With ``lpvalue`` it is pickFDCValue (fieldexamples/outcrop/core/pickFDCValue.m:1-134),
+    which differs only in starting at the given row: idx(lp) = lpvalue (:20)."""
     ml = np.asarray(ml, dtype=np.float64)
     a, b = ml.shape
     if lfg is None:
@@ -383,7 +390,7 @@ def pickFDC1(ml, lp: int, fg: float = 1.5, lfg: Optional[float] = None) -> np.nd
     def col(i):
         return ml[:, i - 1]
 
-    idx[lp] = _nanmax_first(col(lp))
+    idx[lp] = _nanmax_first(col(lp)) if lpvalue is None else int(lpvalue)
     for i in range(lp - 1, 0, -1):
         if idx[i + 1] == a:
             idx[i] = a
@@ -477,6 +484,40 @@ def LHDispPick(ml, vmin, cr_0, method: int, ini: int):
     return res, cr_pre
 
 
+def LHDispPick_field(ml, mlr, vmin, cr_0, res_first: int, res_last: int, start_row: str = "ml"):
+    """The field scripts' LHDispPick (method 1) -> (res, cr_pre).
+
+    Start column ini = the first column holding the largest |mm|
+    (``[~,ini] = max(max(mlr))``).  ``start_row="mlr"`` is
+    fieldexamples/outcrop/core/LHDispPick.m:3-14: pickFDCValue from the peak
+    row of mlr(:, ini); ``start_row="ml"`` is
+    fieldexamples/QadimahFault/core/LHDispPick.m:4-25: pickFDC1 from ini.
+    The misfit is the mean over cr_pre(res_first:end-res_last) (outcrop 15,
+    15 at :21; Qadimah 50, 30 at :25)."""
+    mlr = np.asarray(mlr, dtype=np.float64)
+    colmax = np.array([np.nan if np.all(np.isnan(c)) else np.nanmax(c) for c in mlr.T])
+    ini = _nanmax_first(colmax)
+    if start_row == "mlr":
+        idx = pickFDC1(ml, ini, 2, 4, lpvalue=_nanmax_first(mlr[:, ini - 1]))
+    else:
+        idx = pickFDC1(ml, ini, 2, 4)
+    cr_pre = idx + vmin
+    d = (cr_pre - np.ravel(cr_0))[res_first - 1:cr_pre.size - res_last]
+    with np.errstate(invalid="ignore"):
+        res = float(np.mean(d ** 2)) if d.size else float("nan")
+    return res, cr_pre, ini
+
+
+def norm_trace(data_in) -> np.ndarray:
+    """fieldexamples/outcrop/core/norm_trace.m:4-16 (the QadimahFault copy is
+    the same): each column divided by its max |x|; MATLAB's max skips NaN;
+    a zero column is 0/0 = NaN."""
+    d = np.asarray(data_in, dtype=np.float64)
+    peak = np.fmax.reduce(np.abs(d), axis=0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return d / peak[None, :]
+
+
 # --------------------------------------------------------------------------
 # SoftMax adjoint source (ADWDgrad_1.m)
 # --------------------------------------------------------------------------
@@ -520,8 +561,11 @@ def adjoint_side(sfb, g, beta, nt) -> np.ndarray:
     return inv_dft_real(graddr, ccn, nt)
 
 
-def ADWDgrad_1(nt, ng, ns, npair, is_, w, m, M, SoftArgNorm, grad_outputr, grad_outputl, sfbr, sfbl):
-    """core/ADWDgrad_1.m:1-82 -> a_data_res (nt x ng)."""
+def ADWDgrad_1(nt, ng, ns, npair, is_, w, m, M, SoftArgNorm, grad_outputr, grad_outputl, sfbr, sfbl,
+               peak_norm: bool = False):
+    """core/ADWDgrad_1.m:1-82 -> a_data_res (nt x ng).  ``peak_norm`` is the
+    field copies' extra line 79, a_data_res./max(abs(a_data_res(:)))
+    (fieldexamples/outcrop/core/ADWDgrad_1.m, QadimahFault/core/ADWDgrad_1.m)."""
     a = np.zeros((nt, ng))
     if is_ <= ns - math.floor(m / M) - mround(w / M):
         gr = adjoint_side(sfbr, grad_outputr, SoftArgNorm, nt)
@@ -530,6 +574,9 @@ def ADWDgrad_1(nt, ng, ns, npair, is_, w, m, M, SoftArgNorm, grad_outputr, grad_
         gl = adjoint_side(sfbl, grad_outputl, SoftArgNorm, nt)
         pos = sfbl["uxtpos"][:-1]
         a[:, pos - 1] = gl[::-1][:-1].T if gl.shape[0] > 1 else np.zeros((nt, 0))
+    if peak_norm:
+        with np.errstate(invalid="ignore", divide="ignore"):
+            a = a / np.fmax.reduce(np.abs(a), axis=None)
     return a
 
 
@@ -622,13 +669,32 @@ def swd_line_search(vs, dk, res0, misfit, pur, vsmin, vsmax, f1, f2=float("nan")
 # One SWD shot-gradient (the parfor body SWD.m:171-194) for tests
 # --------------------------------------------------------------------------
 
+def pick_side(ml, sfb, vmin, cr_0, ini, variant=None):
+    """LHDispPick of SWD.m (variant None) or of the field scripts (variant
+    dict with auto_start, res_first, res_last) -> (res, cr_pre)."""
+    v = variant or {}
+    rf, rl = max(int(v.get("res_first", 1)), 1), int(v.get("res_last", 0))
+    if v.get("auto_start"):
+        res, cr, _ = LHDispPick_field(ml, sfb["ml1"], vmin, cr_0, rf, rl, v.get("start_row", "ml"))
+        return res, cr
+    res, cr = LHDispPick(ml, vmin, cr_0, 1, ini)
+    if (rf, rl) != (1, 0):
+        d = (cr - np.ravel(cr_0))[rf - 1:cr.size - rl]
+        res = float(np.mean(d ** 2)) if d.size else float("nan")
+    return res, cr
+
+
 def swd_shot(vp, vs, s, nbc, nt, dx, dt, sx, sz, gx, gz, in_wf, fd_order, is_, ns, radon, cr0_r, cr0_l,
-             beta, ini, w, precision="d"):
+             beta, ini, w, precision="d", variant=None):
     """Forward, RTrAD/RTlAD, LHDispPick, ADWDgrad_1, adjoint for one shot.
 
     ``radon`` = dict(df, fmin, fmax, vmin, vmax, ds, offset, m, M).  Returns a
     dict with seismo_w, picks, residuals, adjoint source and the three images.
+    ``variant`` (dict: norm_trace, auto_start, res_first, res_last,
+    adj_peak_norm) is the field scripts' parfor body
+    (fieldexamples/outcrop/outcrop.m:163-186 with its core/ copies).
     """
+    v = variant or {}
     dtx = dt / dx
     su, sw, snaps = staggerfd_eigend(is_, nbc, nt, dtx, dx, dt, sx, sz, gx, gz, s, vp, vs, 1, fd_order, "w", 0,
                                      in_wf, precision)
@@ -636,23 +702,24 @@ def swd_shot(vp, vs, s, nbc, nt, dx, dt, sx, sz, gx, gz, in_wf, fd_order, is_, n
     ng = np.size(gx)
     np_ = r["vmax"] - r["vmin"] + 1
     out = {"seismo_w": sw}
+    sw_rt = norm_trace(sw) if v.get("norm_trace") else sw
     gr = gl = np.zeros(mround((r["fmax"] - r["fmin"]) / r["df"]) + 1)
     sfbr = sfbl = None
     M, m = r["M"], r["m"]
     if is_ <= ns - math.floor(m / M) - mround(w / M):
-        ml, _, sfbr = rt_side(sw, "r", is_, r["df"], dt, np_, r["vmin"], r["vmax"], r["fmin"], r["fmax"], nt, ng,
+        ml, _, sfbr = rt_side(sw_rt, "r", is_, r["df"], dt, np_, r["vmin"], r["vmax"], r["fmin"], r["fmax"], nt, ng,
                               r["ds"], r["offset"], m, M)
-        out["res_r"], out["cr_r"] = LHDispPick(ml, r["vmin"], cr0_r, 1, ini)
+        out["res_r"], out["cr_r"] = pick_side(ml, sfbr, r["vmin"], cr0_r, ini, v)
         out["ml_r"] = ml
         gr = out["cr_r"] - np.ravel(cr0_r)
     if is_ >= mround(w / M) + math.floor(m / M) + 1:
-        ml, _, sfbl = rt_side(sw, "l", is_, r["df"], dt, np_, r["vmin"], r["vmax"], r["fmin"], r["fmax"], nt, ng,
+        ml, _, sfbl = rt_side(sw_rt, "l", is_, r["df"], dt, np_, r["vmin"], r["vmax"], r["fmin"], r["fmax"], nt, ng,
                               r["ds"], r["offset"], m, M)
-        out["res_l"], out["cr_l"] = LHDispPick(ml, r["vmin"], cr0_l, 1, ini)
+        out["res_l"], out["cr_l"] = pick_side(ml, sfbl, r["vmin"], cr0_l, ini, v)
         out["ml_l"] = ml
         gl = out["cr_l"] - np.ravel(cr0_l)
     npair = gr.size
-    adj = ADWDgrad_1(nt, ng, ns, npair, is_, w, m, M, beta, gr, gl, sfbr, sfbl)
+    adj = ADWDgrad_1(nt, ng, ns, npair, is_, w, m, M, beta, gr, gl, sfbr, sfbl, bool(v.get("adj_peak_norm")))
     out["adj"] = adj
     cl_img, cm_img, illum = e2drtm_eigend(snaps, adj, is_, nbc, nt, dtx, dx, dt, gx, gz, s, vp, vs, 1, fd_order, 0,
                                           in_wf, precision)

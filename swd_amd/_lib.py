@@ -56,6 +56,7 @@ SIGNATURES = {
     "swd_engine_gradient": (_I, [_P, _P, _P, _P, _P, _P]),
     "swd_engine_forward_picks": (_I, [_P, _P, _P, _P, _P]),
     "swd_engine_set_fd_path": (_I, [_P, _I]),
+    "swd_engine_set_variant": (_I, [_P, _P]),
     "swd_engine_run": (_I, [_P, _I]),
     "swd_engine_set_picks": (_I, [_P, _P, _P]),
     "swd_engine_set_tile_rows": (_I, [_P, _I]),
@@ -77,6 +78,7 @@ SIGNATURES = {
     "swd_dispersion_info": (_I, [_P, _P, _I]),
     "swd_dispersion_set_gathers": (_I, [_P, _P]),
     "swd_dispersion_set_observed_picks": (_I, [_P, _P, _P]),
+    "swd_dispersion_set_variant": (_I, [_P, _P]),
     "swd_dispersion_pick": (_I, [_P, _P, _P, _P, _P, _P]),
     "swd_dispersion_adjoint": (_I, [_P, ctypes.c_double, _P, _P]),
     "swd_calib_copy": (_I, [ctypes.c_longlong, _I, _P]),

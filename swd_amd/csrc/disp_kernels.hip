@@ -189,11 +189,80 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GemmArgs g)
 }
 
 // ---------------------------------------------------------------------------
+// Trace peaks of norm_trace (fieldexamples/outcrop/core/norm_trace.m:8-16):
+// peak[shot][g] = max_t |traces[shot][t][g]| over all nt samples.  fmaxf
+// skips NaN samples as MATLAB's max does.  64 traces per block (one per
+// lane, coalesced rows of g), the four waves split t and meet in LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void trace_peak_kernel(const float* __restrict__ traces, int nt, int ng,
+                                                         float* __restrict__ peak)
+{
+    __shared__ float red[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int g = blockIdx.x * 64 + lane;
+    const int shot = blockIdx.y;
+    float m0 = 0.f, m1 = 0.f;
+    if (g < ng) {
+        const float* p = traces + (size_t)shot * nt * ng + g;
+        int t = w;
+        for (; t + 4 < nt; t += 8) {
+            m0 = fmaxf(m0, fabsf(p[(size_t)t * ng]));
+            m1 = fmaxf(m1, fabsf(p[(size_t)(t + 4) * ng]));
+        }
+        if (t < nt) m0 = fmaxf(m0, fabsf(p[(size_t)t * ng]));
+    }
+    red[w][lane] = fmaxf(m0, m1);
+    __syncthreads();
+    if (w == 0 && g < ng)
+        peak[(size_t)shot * ng + g] = fmaxf(fmaxf(red[0][lane], red[1][lane]), fmaxf(red[2][lane], red[3][lane]));
+}
+
+// ---------------------------------------------------------------------------
+// Peak normalisation of each shot's adjoint source (the field scripts'
+// ADWDgrad_1.m:79, a_data_res./max(abs(a_data_res(:)))): kPeakParts blocks
+// per shot reduce strided shares to partial maxima, then every block of the
+// scaling pass combines the partials of its shot and divides its share.  An
+// all-zero source gives 0/0 = NaN, as the division in the script.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void shot_peak_part_kernel(const float* __restrict__ a, size_t n,
+                                                             float* __restrict__ part)
+{
+    __shared__ float red[4];
+    const float* p = a + (size_t)blockIdx.y * n;
+    float m = 0.f;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)kPeakParts * 256)
+        m = fmaxf(m, fabsf(p[i]));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        part[(size_t)blockIdx.y * kPeakParts + blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+__global__ __launch_bounds__(256) void shot_peak_scale_kernel(float* __restrict__ a, size_t n,
+                                                              const float* __restrict__ part)
+{
+    __shared__ float pk;
+    if (threadIdx.x < 64) {
+        float m = part[(size_t)blockIdx.y * kPeakParts + threadIdx.x];
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+        if (threadIdx.x == 0) pk = m;
+    }
+    __syncthreads();
+    const float q = pk;
+    float* p = a + (size_t)blockIdx.y * n;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)kPeakParts * 256) p[i] = p[i] / q;
+}
+
+// ---------------------------------------------------------------------------
 // Window gather: X[k][l][col] = D(shot, k, trace(col, l)), 0 beyond L(col).
 // D layout: D[shot][k2][g] with Re at k2 = k and Im at k2 = npair + k.
+// With trace peaks (norm_trace), each spectrum is divided by its trace's
+// peak: the DFT is linear, so this is the spectrum of the normalised trace.
+// A zero trace gives 0/0 = NaN, as norm_trace does.
 // ---------------------------------------------------------------------------
 __global__ void radon_gather_kernel(const float* D, const DispCol* cols, int ncol, int npair, int ng, int Lmax,
-                                    float2* X)
+                                    const float* peak, float2* X)
 {
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const int l = blockIdx.y;
@@ -206,6 +275,11 @@ __global__ void radon_gather_kernel(const float* D, const DispCol* cols, int nco
         const float* d = D + (size_t)c.shot * 2 * npair * ng;
         v.x = d[(size_t)k * ng + g];
         v.y = d[(size_t)(npair + k) * ng + g];
+        if (peak) {
+            const float q = peak[(size_t)c.shot * ng + g];
+            v.x = v.x / q;
+            v.y = v.y / q;
+        }
     }
     X[((size_t)k * Lmax + l) * ncol + col] = v;
 }
@@ -358,9 +432,12 @@ __device__ bool is_local_min(const float* v, int lo, int hi, int p)
     return yn > y;
 }
 
-__global__ __launch_bounds__(64) void pick_kernel(const float* ml, const DispCol* cols, int np, int npair,
-                                                  int lp, float fg, float lfg, double vmin, const double* cr0,
-                                                  double* cr_pre, double* res, int* status)
+// lp = 0 picks from the column of the spectrum's largest |mm| (the field
+// scripts' LHDispPick: [~,ini] = max(max(mlr))).  The misfit averages the
+// columns r0..npair-1-r1 (0-based; the field scripts' cr_pre(a:end-b)).
+__global__ __launch_bounds__(64) void pick_kernel(const float* ml, const float* mlr, const DispCol* cols, int np,
+                                                  int npair, int lp, float fg, float lfg, double vmin, int r0, int r1,
+                                                  const double* cr0, double* cr_pre, double* res, int* status)
 {
     const int col = blockIdx.x;
     const int lane = threadIdx.x;
@@ -410,6 +487,19 @@ __global__ __launch_bounds__(64) void pick_kernel(const float* ml, const DispCol
         return 0;
     };
 
+    if (lp == 0) {
+        // column maxima of |mm| (NaN ignored), then the first column holding the largest
+        const float* u = mlr + (size_t)col * npair * np;
+        float best = -INFINITY;
+        int bk = 0;
+        for (int k = 0; k < npair; ++k) {
+            float m = -INFINITY;
+            for (int j = lane; j < np; j += 64) m = fmaxf(m, u[(size_t)k * np + j]);
+            for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+            if (m > best) { best = m; bk = k; }
+        }
+        lp = bk + 1;
+    }
     const int L1 = lp - 1;  // 0-based index of lp
     if (lane == 0) for (int i = 0; i < npair; ++i) { chz[i] = 1; qj[i] = NAN; dvmax[i] = 0; dvmin[i] = 0; }
     __syncthreads();
@@ -486,9 +576,9 @@ __global__ __launch_bounds__(64) void pick_kernel(const float* ml, const DispCol
             const double v = fidx[i] + vmin;
             cr_pre[(size_t)col * npair + i] = v;
             const double d = v - cr0[(size_t)col * npair + i];
-            acc += d * d;
+            if (i >= r0 && i < npair - r1) acc += d * d;
         }
-        res[col] = acc / npair;
+        res[col] = acc / (double)max(npair - r1 - r0, 0);  // mean of an empty window is 0/0 = NaN
         status[col] = st;
     }
 }
@@ -612,10 +702,29 @@ int launch_gemm_f32(hipStream_t st, const GemmArgs& g, int batch)
     return kOk;
 }
 
-int launch_radon_gather(hipStream_t st, const float* D, const DispCol* cols, int ncol, int npair, int ng, int Lmax,
-                        float2* X)
+int launch_shot_peak_norm(hipStream_t st, float* a, size_t n, int nshots, float* part)
 {
-    radon_gather_kernel<<<dim3(div_up(ncol, 64), Lmax, npair), 64, 0, st>>>(D, cols, ncol, npair, ng, Lmax, X);
+    static_assert(kPeakParts == 64, "the scaling pass combines the partials with one wavefront");
+    if (n == 0 || nshots <= 0) return kOk;
+    shot_peak_part_kernel<<<dim3(kPeakParts, nshots), 256, 0, st>>>(a, n, part);
+    SWD_HIP(hipGetLastError());
+    shot_peak_scale_kernel<<<dim3(kPeakParts, nshots), 256, 0, st>>>(a, n, part);
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
+int launch_trace_peak(hipStream_t st, const float* traces, int nt, int ng, int nshots, float* peak)
+{
+    if (nt <= 0 || ng <= 0 || nshots <= 0) return kOk;
+    trace_peak_kernel<<<dim3(div_up(ng, 64), nshots), 256, 0, st>>>(traces, nt, ng, peak);
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
+int launch_radon_gather(hipStream_t st, const float* D, const DispCol* cols, int ncol, int npair, int ng, int Lmax,
+                        const float* peak, float2* X)
+{
+    radon_gather_kernel<<<dim3(div_up(ncol, 64), Lmax, npair), 64, 0, st>>>(D, cols, ncol, npair, ng, Lmax, peak, X);
     SWD_HIP(hipGetLastError());
     return kOk;
 }
@@ -637,10 +746,12 @@ int launch_radon_abs(hipStream_t st, const float2* MM, int np, int npair, int nc
     return kOk;
 }
 
-int launch_pick(hipStream_t st, const float* ml, const DispCol* cols, int ncol, int np, int npair, int lp, float fg,
-                float lfg, double vmin, const double* cr0, double* cr_pre, double* res, int* status)
+int launch_pick(hipStream_t st, const float* ml, const float* mlr, const DispCol* cols, int ncol, int np, int npair,
+                int lp, float fg, float lfg, double vmin, int r0, int r1, const double* cr0, double* cr_pre,
+                double* res, int* status)
 {
-    pick_kernel<<<ncol, 64, 0, st>>>(ml, cols, np, npair, lp, fg, lfg, vmin, cr0, cr_pre, res, status);
+    if (lp == 0 && !mlr) { set_error("an automatic pick start needs |mm|"); return kErrArg; }
+    pick_kernel<<<ncol, 64, 0, st>>>(ml, mlr, cols, np, npair, lp, fg, lfg, vmin, r0, r1, cr0, cr_pre, res, status);
     SWD_HIP(hipGetLastError());
     return kOk;
 }

@@ -82,6 +82,15 @@ __global__ void model_step_kernel(const double* __restrict__ vs, const double* _
     out[k] = v;
 }
 
+// vp = ratio * vs (the field scripts' vp = vs1*1.732 after every model
+// change, fieldexamples/outcrop/outcrop.m:217,245,271,301,331), in FP64.
+__global__ void model_scale_kernel(const double* __restrict__ vs, double ratio, double* __restrict__ vp, size_t n)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    vp[k] = vs[k] * ratio;
+}
+
 // One workgroup, fixed summation order (deterministic): out[0] = min(a, b),
 // out[1] = max(a, b), out[2] = sum a^2, out[3] = sum b^2.  NaNs propagate
 // into the sums; min / max skip them.
@@ -141,6 +150,13 @@ int launch_model_step(hipStream_t st, const double* vs, const double* dk, double
                       double* out, size_t n)
 {
     model_step_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(vs, dk, step, vsmin, vsmax, out, n);
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
+int launch_model_scale(hipStream_t st, const double* vs, double ratio, double* vp, size_t n)
+{
+    model_scale_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(vs, ratio, vp, n);
     SWD_HIP(hipGetLastError());
     return kOk;
 }

@@ -43,12 +43,22 @@ int launch_gemm_f32(hipStream_t st, const GemmArgs& g, int batch);
 // 1 = BF16 split precision (three bf16 MFMA products per FP32 product).
 int radon_precision();
 int set_radon_precision(int mode);
+// a[shot][*] /= max |a[shot][*]| over each shot's n values (NaN skipped);
+// part holds nshots x kPeakParts partial maxima
+constexpr int kPeakParts = 64;
+int launch_shot_peak_norm(hipStream_t st, float* a, size_t n, int nshots, float* part);
+// peak[shot][g] = max_t |traces[shot][t][g]| (norm_trace's divisor)
+int launch_trace_peak(hipStream_t st, const float* traces, int nt, int ng, int nshots, float* peak);
+// peak may be null (no trace normalisation)
 int launch_radon_gather(hipStream_t st, const float* D, const DispCol* cols, int ncol, int npair, int ng, int Lmax,
-                        float2* X);
+                        const float* peak, float2* X);
 int launch_slant_stack(hipStream_t st, const SlantArgs& a, bool adjoint);
 int launch_radon_abs(hipStream_t st, const float2* MM, int np, int npair, int ncol, float* ml, float* mlr);
-int launch_pick(hipStream_t st, const float* ml, const DispCol* cols, int ncol, int np, int npair, int lp, float fg,
-                float lfg, double vmin, const double* cr0, double* cr_pre, double* res, int* status);
+// lp = 0: start at the column of the largest |mm| (mlr); the misfit
+// averages columns r0 .. npair-1-r1
+int launch_pick(hipStream_t st, const float* ml, const float* mlr, const DispCol* cols, int ncol, int np, int npair,
+                int lp, float fg, float lfg, double vmin, int r0, int r1, const double* cr0, double* cr_pre,
+                double* res, int* status);
 int launch_argmax_pick(hipStream_t st, const float* ml, int np, int npair, double vmin, const double* cr0,
                        double* cr_pre);
 int launch_softargmax(hipStream_t st, const float* mlr, const float2* MM, const DispCol* cols, int np, int npair,

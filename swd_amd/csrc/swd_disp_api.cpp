@@ -3,6 +3,7 @@
 // LHDispPick and ADWDgrad_1 of the C ABI.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -94,6 +95,8 @@ int DispPlan::init(const DispGeom& geom, const std::vector<DispCol>& columns, hi
     if ((rc = cr0.alloc(nc * npair * sizeof(double)))) return rc;
     if ((rc = res.alloc(nc * sizeof(double)))) return rc;
     if ((rc = status.alloc(nc * sizeof(int)))) return rc;
+    if ((rc = peak.alloc((size_t)g.nshots * g.ng * sizeof(float)))) return rc;
+    if ((rc = peak_part.alloc((size_t)g.nshots * kPeakParts * sizeof(float)))) return rc;
     SWD_HIP(hipMemsetAsync(cr0.p, 0, cr0.bytes, st));
     SWD_HIP(hipMemsetAsync(res.p, 0, res.bytes, st));
     SWD_HIP(hipMemsetAsync(status.p, 0, status.bytes, st));
@@ -101,10 +104,14 @@ int DispPlan::init(const DispGeom& geom, const std::vector<DispCol>& columns, hi
     return kOk;
 }
 
-// R1-R3: DFT at the bins, window gather, slant stack, |mm| and normalisation
+// R1-R3: DFT at the bins, window gather, slant stack, |mm| and normalisation.
+// With norm_traces the gathers are first divided trace by trace by their peak
+// |x| (norm_trace on the predicted data, fieldexamples/outcrop/outcrop.m:164):
+// the peaks come from the traces, the division is applied to the spectra.
 int DispPlan::forward(const float* traces)
 {
     int rc;
+    if (norm_traces && (rc = launch_trace_peak(st, traces, g.nt, g.ng, g.nshots, peak.as<float>()))) return rc;
     GemmArgs a;
     memset(&a, 0, sizeof a);
     a.A = W.as<float>();                 // A(m = k2, k = t) = W[t][k2]
@@ -115,7 +122,8 @@ int DispPlan::forward(const float* traces)
     a.sCm = g.ng; a.sCn = 1; a.sCb = 2LL * g.npair * g.ng;
     a.M = 2 * g.npair; a.N = g.ng; a.K = g.nt_eff; a.alpha = 1.f;
     if ((rc = launch_gemm_f32(st, a, g.nshots))) return rc;
-    if ((rc = launch_radon_gather(st, D.as<float>(), dcols.as<DispCol>(), ncol, g.npair, g.ng, Lmax, X.as<float2>())))
+    if ((rc = launch_radon_gather(st, D.as<float>(), dcols.as<DispCol>(), ncol, g.npair, g.ng, Lmax,
+                                  norm_traces ? peak.as<float>() : nullptr, X.as<float2>())))
         return rc;
     SlantArgs s;
     s.in = X.as<float2>(); s.out = MM.as<float2>();
@@ -127,9 +135,11 @@ int DispPlan::forward(const float* traces)
 
 int DispPlan::pick(int lp, float fg, float lfg)
 {
-    if (lp < 1 || lp > g.npair) { set_error("pick start column %d outside 1..%d", lp, g.npair); return kErrArg; }
-    return launch_pick(st, ml.as<float>(), dcols.as<DispCol>(), ncol, g.np, g.npair, lp, fg, lfg, g.vmin,
-                       cr0.as<double>(), crpre.as<double>(), res.as<double>(), status.as<int>());
+    if (auto_start) lp = 0;
+    else if (lp < 1 || lp > g.npair) { set_error("pick start column %d outside 1..%d", lp, g.npair); return kErrArg; }
+    const int r0 = std::max(res_first, 1) - 1, r1 = std::max(res_last, 0);
+    return launch_pick(st, ml.as<float>(), mlr.as<float>(), dcols.as<DispCol>(), ncol, g.np, g.npair, lp, fg, lfg,
+                       g.vmin, r0, r1, cr0.as<double>(), crpre.as<double>(), res.as<double>(), status.as<int>());
 }
 
 // G1-G4: soft-argmax backward + phase, adjoint slant stack, placement, inverse DFT
@@ -157,7 +167,10 @@ int DispPlan::adjoint(double beta, float* adj_out)
     a.C = adj_out;                       // C[shot][t][g]
     a.sCm = g.ng; a.sCn = 1; a.sCb = (long long)g.nt * g.ng;
     a.M = g.nt; a.N = g.ng; a.K = 2 * g.npair; a.alpha = (float)(1.0 / g.ccn);
-    return launch_gemm_f32(st, a, g.nshots);
+    if ((rc = launch_gemm_f32(st, a, g.nshots))) return rc;
+    // a_data_res./max(abs(a_data_res(:))) of the field scripts' ADWDgrad_1 (line 79)
+    if (adj_peak_norm) return launch_shot_peak_norm(st, adj_out, (size_t)g.nt * g.ng, g.nshots, peak_part.as<float>());
+    return kOk;
 }
 
 // ---------------------------------------------------------------------------
@@ -220,8 +233,8 @@ static int lhdisppick_impl(const float* ml, int np, int npair, double vmin, cons
         return rc;
     if (method == 1) {
         if (ini < 1 || ini > npair) { set_error("ini %d outside 1..%d", ini, npair); return kErrArg; }
-        if ((rc = launch_pick(st.s, dml.as<float>(), dcols.as<DispCol>(), 1, np, npair, ini, 2.f, 4.f, vmin,
-                              dcr0.as<double>(), dpre.as<double>(), dres.as<double>(), dstat.as<int>())))
+        if ((rc = launch_pick(st.s, dml.as<float>(), nullptr, dcols.as<DispCol>(), 1, np, npair, ini, 2.f, 4.f, vmin,
+                              0, 0, dcr0.as<double>(), dpre.as<double>(), dres.as<double>(), dstat.as<int>())))
             return rc;
     } else {
         if ((rc = launch_argmax_pick(st.s, dml.as<float>(), np, npair, vmin, dcr0.as<double>(), dpre.as<double>())))

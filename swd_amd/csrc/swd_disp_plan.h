@@ -4,6 +4,7 @@
 
 #include <vector>
 
+#include "swd_abi.h"
 #include "swd_disp.h"
 #include "swd_host.h"
 
@@ -39,11 +40,27 @@ struct DispPlan {
     DevBuf res;     // [ncol] mean squared pick misfit
     DevBuf status;  // [ncol] pick status
     DevBuf dcols;
+    DevBuf peak;       // [shot][g] trace peaks max_t |x| (norm_trace)
+    DevBuf peak_part;  // [shot][kPeakParts] partial peaks of the adjoint sources
+    // variants of the field-data scripts (all off: SWD.m)
+    bool norm_traces = false;    // divide each trace by its peak before the Radon
+    bool auto_start = false;     // pick from the column of the largest |mm|
+    int res_first = 1, res_last = 0;  // misfit over cr(res_first : end - res_last), 1-based
+    bool adj_peak_norm = false;  // divide each shot's adjoint source by its peak |a|
 
     int init(const DispGeom& geom, const std::vector<DispCol>& columns, hipStream_t stream);
     int forward(const float* traces);             // traces [shot][t][g]
     int pick(int lp, float fg, float lfg);
     int adjoint(double beta, float* adj_out);     // adj_out [shot][t][g]
 };
+
+inline void set_plan_variant(DispPlan& d, const swd_variant& v)
+{
+    d.norm_traces = v.norm_trace != 0;
+    d.auto_start = v.auto_start != 0;
+    d.res_first = v.res_first > 1 ? v.res_first : 1;
+    d.res_last = v.res_last > 0 ? v.res_last : 0;
+    d.adj_peak_norm = v.adj_peak_norm != 0;
+}
 
 }  // namespace swd

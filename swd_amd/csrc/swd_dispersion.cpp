@@ -122,6 +122,19 @@ int swd_dispersion_set_gathers(swd_dispersion* h, const float* gathers)
     return kOk;
 }
 
+int swd_dispersion_set_variant(swd_dispersion* h, const swd_variant* v)
+{
+    if (!h || !v) { set_error("null argument"); return kErrArg; }
+    Dispersion& d = h->d;
+    if (v->res_first < 0 || v->res_last < 0 || (v->res_first > 1 ? v->res_first - 1 : 0) + v->res_last >= d.disp.g.npair) {
+        set_error("bad misfit window cr(%d:end-%d) for %d columns", v->res_first, v->res_last, d.disp.g.npair);
+        return kErrArg;
+    }
+    set_plan_variant(d.disp, *v);
+    d.have_picks = false;
+    return kOk;
+}
+
 int swd_dispersion_set_observed_picks(swd_dispersion* h, const double* cr0_r, const double* cr0_l)
 {
     if (!h) return kErrArg;

@@ -174,6 +174,7 @@ int Engine::set_vs_device(const double* vs_dev)
     if (!have_model) { set_error("set_model must run once first (it fixes vp)"); return kErrArg; }
     int rc;
     double h[4];
+    if (vp_ratio > 0 && (rc = launch_model_scale(st, vs_dev, vp_ratio, vp_d.as<double>(), splane))) return rc;
     if ((rc = launch_model_stats(st, vp_d.as<double>(), vs_dev, splane, stats.as<double>()))) return rc;
     SWD_HIP(hipMemcpyAsync(h, stats.p, sizeof h, hipMemcpyDeviceToHost, st));
     SWD_HIP(hipStreamSynchronize(st));
@@ -571,6 +572,21 @@ int swd_engine_read_model(swd_engine* e, double* vs)
     Engine& g = e->e;
     SWD_HIP(hipMemcpyAsync(vs, g.vs_d.p, g.splane * sizeof(double), hipMemcpyDeviceToHost, g.st));
     SWD_HIP(hipStreamSynchronize(g.st));
+    return kOk;
+}
+
+int swd_engine_set_variant(swd_engine* e, const swd_variant* v)
+{
+    if (!e || !v) { set_error("null argument"); return kErrArg; }
+    if (!(v->vp_ratio >= 0) || std::isinf(v->vp_ratio)) { set_error("vp_ratio must be finite and >= 0"); return kErrArg; }
+    if (v->res_first < 0 || v->res_last < 0) { set_error("negative misfit window"); return kErrArg; }
+    Engine& g = e->e;
+    if ((v->res_first > 1 ? v->res_first - 1 : 0) + v->res_last >= g.npair) {
+        set_error("misfit window cr(%d:end-%d) is empty for %d columns", v->res_first, v->res_last, g.npair);
+        return kErrArg;
+    }
+    set_plan_variant(g.disp, *v);
+    g.vp_ratio = v->vp_ratio;
     return kOk;
 }
 

@@ -43,6 +43,7 @@ public:
     int tb_rows[2] = {0, 0};   // band rows of the time-blocked march, forward / adjoint (0: from occupancy)
     bool have_model = false, have_wavelet = false;
     double vpmax = 0;
+    double vp_ratio = 0;  // > 0: device model updates set vp = vp_ratio * vs (field scripts); 0: vp fixed (SWD.m)
 
     DevBuf coefs;     // 6 planes float: temp, b, b1, ca, cl, cm1
     DevBuf vs_d;      // nz x nx double (MATLAB layout) for the assembly
@@ -70,7 +71,8 @@ public:
     int set_model(const double* vp, const double* vs);
     int set_wavelet(const double* s);
     int set_observed_picks(const double* cr0_r, const double* cr0_l);
-    // rebuild the coefficient planes for a device-resident vs (vp unchanged)
+    // rebuild the coefficient planes for a device-resident vs (vp unchanged,
+    // or vp = vp_ratio * vs when vp_ratio > 0)
     int set_vs_device(const double* vs_dev);
     // one SWD iteration (SWD.m:171-335): gradient, smoothing, line search,
     // model update, all on the device

@@ -36,6 +36,7 @@ int launch_prep_coefs(hipStream_t st, const double* vp, const double* vs, const 
 int launch_smooth2a(hipStream_t st, const double* in, int nz, int nx, int Nr, int Nc, double* out);
 int launch_model_step(hipStream_t st, const double* vs, const double* dk, double step, double vsmin, double vsmax,
                       double* out, size_t n);
+int launch_model_scale(hipStream_t st, const double* vs, double ratio, double* vp, size_t n);
 int launch_model_stats(hipStream_t st, const double* a, const double* b, size_t n, double* out);
 
 // First-fit sub-allocator over one allocation made once (device memory via

@@ -155,6 +155,13 @@ class DispersionBatch:
             raise ValueError(f"gathers must be {(self.nshots, q['nt'], q['ng'])}, got {g.shape}")
         check(load().swd_dispersion_set_gathers(self.h, ptr(g)))
 
+    def set_variant(self, variant="swd"):
+        """Field-script variant (swd_dispersion_set_variant; vp_ratio ignored)."""
+        import ctypes
+        from .field import variant_struct
+        v = variant_struct(variant)
+        check(load().swd_dispersion_set_variant(self.h, ctypes.byref(v)))
+
     def set_observed_picks(self, cr0_r, cr0_l):
         check(load().swd_dispersion_set_observed_picks(self.h, ptr(fortran(cr0_r, np.float64)),
                                                         ptr(fortran(cr0_l, np.float64))))

@@ -156,6 +156,15 @@ class Engine:
         b = None if cr_l is None else fortran(cr_l, np.float64)
         check(lib().swd_engine_set_picks(self.h, ptr(a), ptr(b)))
 
+    def set_variant(self, variant="swd"):
+        """Field-script variant of the dispersion passes and the model update
+        (swd_engine_set_variant): a name of :data:`swd_amd.field.VARIANTS`
+        or a dict of its fields."""
+        from .field import variant_struct
+        v = variant_struct(variant)
+        check(lib().swd_engine_set_variant(self.h, ctypes.byref(v)))
+        self.variant = v
+
     def set_fd_path(self, mode: int) -> int:
         return lib().swd_engine_set_fd_path(self.h, int(mode))
 

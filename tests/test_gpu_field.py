@@ -1,0 +1,235 @@
+"""The field-script variants on the GPU (swd_engine_set_variant /
+swd_dispersion_set_variant) against the oracle: norm_trace on gathers with
+uneven trace gains, the strongest-column pick start, the windowed misfit, the
+peak-normalised adjoint sources, the outcrop survey's shot-gradient and a
+device iteration with vp tied to vs.  FP32 results within relative L2 1e-4
+of FP64, picks within one velocity bin."""
+import numpy as np
+import pytest
+
+from oracle import swd_oracle as O
+from swd_amd import field as F
+from swd_amd import problems as P
+from swd_amd.dispersion import DispersionBatch
+from swd_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+NSHOT = 2
+OUTCROP = dict(F.VARIANTS["outcrop"])
+
+
+def rel(a, b):
+    a, b = np.ravel(a), np.ravel(b)
+    assert np.array_equal(np.isnan(a), np.isnan(b))
+    m = np.isfinite(b)
+    return float(np.linalg.norm(a[m] - b[m]) / max(np.linalg.norm(b[m]), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def gained_chirp():
+    prob, _ = P.production_shots(nshots=NSHOT)
+    g = P.chirp_gathers(nshots=NSHOT, nt=prob["nt"], ng=prob["ng"], dt=prob["dt"], dx=prob["ds"], sx=prob["sx"])
+    gains = (10.0 ** np.random.default_rng(7).uniform(-2, 2, (NSHOT, 1, prob["ng"]))).astype(np.float32)
+    return prob, g, (g * gains).astype(np.float32)
+
+
+def _radon_args(q, s, np_):
+    return (s + 1, q["df"], q["dt"], np_, q["vmin"], q["vmax"], q["fmin"], q["fmax"], q["nt"], q["ng"], q["ds"],
+            q["offset"], q["m"], q["M"])
+
+
+def test_dispersion_batch_outcrop_variant_vs_oracle(gained_chirp):
+    prob, g, gg = gained_chirp
+    q = prob
+    D = DispersionBatch(prob)
+    try:
+        D.set_variant("outcrop")
+        D.set_gathers(gg)
+        ones = np.ones((D.npair, NSHOT))
+        D.set_observed_picks(ones, ones)
+        picks = D.pick()
+        D.set_gathers(g)                       # the same traces without the gains
+        p0 = D.pick()
+        r = np.random.default_rng(11)
+        gr = np.round(r.normal(0, 5, (D.npair, NSHOT)))
+        gl = np.round(r.normal(0, 5, (D.npair, NSHOT)))
+        D.set_gathers(gg)
+        D.set_observed_picks(picks["cr_r"] - gr, picks["cr_l"] - gl)
+        again = D.pick()
+        adj, _ = D.adjoint(10.0)
+    finally:
+        D.close()
+    for s in range(NSHOT):
+        xn = O.norm_trace(gg[s].astype(np.float64))
+        sides = {}
+        for side, key in (("r", "cr_r"), ("l", "cr_l")):
+            ml, L, sfb = O.rt_side(xn, side, *_radon_args(q, s, D.np))
+            sides[side] = sfb
+            if L < 200:
+                continue               # short windows: near-flat spectra whose picks move with any rounding
+            res, cr, ini = O.LHDispPick_field(ml, sfb["ml1"], q["vmin"], np.ones(D.npair), 15, 15, "ml")
+            _, cr_lit, _ = O.LHDispPick_field(ml, sfb["ml1"], q["vmin"], np.ones(D.npair), 15, 15, "mlr")
+            assert np.array_equal(cr, cr_lit)  # outcrop's pickFDCValue start row == pickFDC1's here
+            gpu = picks[key][:, s]
+            print(f"shot {s + 1} side {side}: start column {ini}, max pick difference {np.max(np.abs(gpu - cr))}")
+            assert np.max(np.abs(gpu - cr)) <= 1.0, (s, side)
+            assert np.max(np.abs(p0[key][:, s] - gpu)) <= 1.0      # trace gains drop out
+            if np.array_equal(gpu, cr):
+                rk = "res_r" if side == "r" else "res_l"
+                assert picks[rk][s] == pytest.approx(res, rel=1e-12)
+        # adjoint sources: the picks enter only through g = cr_pre - cr_0, fixed above
+        assert np.array_equal(again["cr_r"][:, s], picks["cr_r"][:, s])
+        assert np.array_equal(again["cr_l"][:, s], picks["cr_l"][:, s])
+        a_ref = O.ADWDgrad_1(q["nt"], q["ng"], NSHOT, D.npair, s + 1, q["w"], q["m"], q["M"], 10.0, gr[:, s],
+                             gl[:, s], sides["r"], sides["l"], peak_norm=True)
+        e = rel(adj[s], a_ref)
+        print(f"shot {s + 1}: peak-normalised adjoint source rel-L2 vs FP64 {e:.3e}, peak {np.max(np.abs(adj[s]))}")
+        assert np.max(np.abs(adj[s])) == pytest.approx(1.0, abs=1e-6)
+        assert e < 1e-4
+
+
+def _outcrop_shots(shots, nt):
+    prob, mod = F.outcrop(offset=50, nt=nt)
+    idx = np.asarray(shots)
+    prob["sx"] = 1 + 4 * (idx - 1)
+    prob["sz"] = np.ones(idx.size, dtype=int)
+    prob["is_index"] = idx
+    vs_true = mod["vs"] * (1 + 0.08 * np.exp(-((np.arange(240)[None, :] - 120) ** 2 +
+                                                (np.arange(40)[:, None] - 15.0) ** 2) / (2 * 12.0 ** 2)))
+    mod.update(vs_true=vs_true, vp_true=1.732 * vs_true)
+    return prob, mod
+
+
+def test_outcrop_shot_gradient_vs_oracle():
+    """Two outcrop shots with both branches (is = 5, 30 of 60), offset 50,
+    npair 651, the outcrop variant; observed picks from the engine on a
+    perturbed model, shared with the oracle.  Picks within one bin; the
+    adjoint sources and dk_vs then from the oracle's picks (set between the
+    stages), so the FP32 chain downstream of the integer picks is judged
+    against FP64 whatever the pick rounding did."""
+    prob, mod = _outcrop_shots([5, 30], nt=3000)
+    q = prob
+    gx = q["gx0"] + q["dg"] * np.arange(q["ng"], dtype=float)
+    gz = np.full(q["ng"], float(q["gz"]))
+    radon = dict(df=q["df"], fmin=q["fmin"], fmax=q["fmax"], vmin=q["vmin"], vmax=q["vmax"], ds=q["ds"],
+                 offset=q["offset"], m=q["m"], M=q["M"])
+    e = Engine(prob)
+    try:
+        e.set_variant("outcrop")
+        e.set_wavelet(mod["s"])
+        e.set_model(mod["vp_true"], mod["vs_true"])
+        cr0_r, cr0_l = e.observe()
+        e.set_model(mod["vp"], mod["vs"])
+        gpu = e.gradient()
+        gath = e.read("gathers")
+        shots = []
+        for k, is_ in enumerate(q["is_index"]):
+            o = O.swd_shot(mod["vp"], mod["vs"], mod["s"], q["nbc"], q["nt"], q["dx"], q["dt"], q["sx"][k],
+                           q["sz"][k], gx, gz, q["in_wf"], 22, int(is_), q["ns"], radon, cr0_r[:, k], cr0_l[:, k],
+                           q["beta"], q["ini"], q["w"], variant=OUTCROP)
+            shots.append(o)
+            eg = rel(gath[k], o["seismo_w"])
+            dr = np.max(np.abs(gpu["cr_r"][:, k] - o["cr_r"]))
+            dl = np.max(np.abs(gpu["cr_l"][:, k] - o["cr_l"]))
+            print(f"shot {is_}: gathers rel-L2 {eg:.2e}, pick differences right {dr} left {dl}")
+            assert eg < 1e-4 and dr <= 1 and dl <= 1
+            if dr == 0:
+                assert gpu["res_r"][k] == pytest.approx(o["res_r"], rel=1e-9)
+        # downstream of the picks, given the oracle's
+        e.run("forward")
+        e.run("dispersion")
+        e.set_picks(np.array([o["cr_r"] for o in shots]).T, np.array([o["cr_l"] for o in shots]).T)
+        e.run("adjoint_source")
+        e.run("adjoint")
+        e.run("assemble")
+        adj = e.read("adjoint_sources")
+        dk_gpu = e.read("gradient")[0]
+    finally:
+        e.close()
+    for k, o in enumerate(shots):
+        ea = rel(adj[k], o["adj"])
+        print(f"shot {q['is_index'][k]}: peak-normalised adjoint source rel-L2 {ea:.2e}")
+        assert ea < 1e-4 and np.max(np.abs(adj[k])) == pytest.approx(1.0, abs=1e-6)
+    imgs = sum(np.array([o["cl_img"], o["cm_img"], o["illum"]]) for o in shots)
+    ed = rel(dk_gpu, O.assemble_dk_vs(mod["vs"], *imgs))
+    print(f"dk_vs rel-L2 vs FP64: {ed:.2e}")
+    assert ed < 1e-4
+
+
+def test_iteration_with_vp_tied_to_vs_equals_host_restatement():
+    """swd_engine_iterate with vp_ratio 1.732 (and the rest of the outcrop
+    variant) against the oracle's line search driving a second engine whose
+    host path is given vp = 1.732 vs1 for every trial."""
+    prob, mod = P.plain_single_shot(nt=3000)
+    vsmin, vsmax = float(mod["vs_true"].min()), float(mod["vs_true"].max())
+    var = dict(OUTCROP)
+
+    def engine(cr0=None):
+        e = Engine(prob)
+        e.set_variant(var)
+        e.set_wavelet(mod["s"])
+        if cr0 is None:
+            e.set_model(mod["vp_true"], mod["vs_true"])
+            cr0 = e.observe()
+        else:
+            e.set_observed_picks(*cr0)
+        e.set_model(1.732 * mod["vs"], mod["vs"])
+        return e, cr0
+
+    A, cr0 = engine()
+    B, _ = engine(cr0)
+    try:
+        r = A.iterate(vsmin, vsmax, pur=-0.05, smooth_z=1, smooth_x=2, state={"f1": 0.5})
+        vs_a = A.model()
+        dk_a = A.read("smoothed_gradient")
+        g = B.gradient()
+        res0 = float(g["res_r"][0])
+        assert res0 == r["residual"]
+        assert np.allclose(dk_a, O.smooth2a(g["dk_vs"], 1, 2), rtol=1e-12, atol=0)
+        vs0 = np.asfortranarray(mod["vs"], dtype=np.float64)
+
+        def misfit(vs1):
+            B.set_model(vs1 * 1.732, vs1)
+            return float(B.forward_picks()["res_r"][0])
+
+        vs_b, rec = O.swd_line_search(vs0, dk_a, res0, misfit, -0.05, vsmin, vsmax, 0.5, alpha=r["alpha"])
+        print("device:", {k: r[k] for k in ("residual", "res0", "alpha", "gama", "forwards")}, "host:", rec)
+        assert rec["forwards"] == r["forwards"] and rec["gama"] == r["gama"] and rec["res0"] == r["res0"]
+        np.testing.assert_array_equal(vs_a, vs_b)
+        A.forward_picks()                        # the accepted model's planes, vp = 1.732 vs
+        B.set_model(vs_a * 1.732, vs_a)
+        B.forward_picks()
+        np.testing.assert_array_equal(A.read("gathers"), B.read("gathers"))
+    finally:
+        A.close()
+        B.close()
+
+
+def test_field_inversion_with_given_observed_picks():
+    """invert() as the outcrop script drives it: the outcrop variant, vp tied
+    to vs, the script's vs bounds, offset step 5, and observed picks handed
+    in per offset (as manual_picks gives them from a cube)."""
+    from swd_amd.inversion import invert
+    prob, mod = _outcrop_shots([5, 10, 15, 20], nt=3000)
+    picks = {}
+
+    def observed(offset, npair):
+        if offset not in picks:
+            q = dict(prob, offset=offset)
+            e = Engine(q)
+            try:
+                e.set_variant("outcrop")
+                e.set_wavelet(mod["s"])
+                e.set_model(mod["vp_true"], mod["vs_true"])
+                picks[offset] = e.observe()
+            finally:
+                e.close()
+        assert picks[offset][0].shape == (npair, 4)
+        return picks[offset]
+
+    out = invert(prob, mod, iterations=2, variant="outcrop", observed=observed, vs_bounds=F.OUTCROP_VS_BOUNDS,
+                 offset_step=5, min_offset=10, log=print)
+    assert np.all(np.isfinite(out["vs"]))
+    assert out["vs"].min() >= F.OUTCROP_VS_BOUNDS[0] and out["vs"].max() <= F.OUTCROP_VS_BOUNDS[1]
+    assert out["records"][-1]["res0"] <= out["residual"][0]

@@ -1,0 +1,129 @@
+"""The oracle's field-script variants pinned on closed forms and hand-built
+spectra (norm_trace, the field LHDispPick, the peak-normalised ADWDgrad_1),
+and the host side of the field-data inputs (variants, manual-pick slicing,
+the outcrop geometry).  CPU only."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import swd_oracle as O
+from swd_amd import field as F
+
+OUTCROP_DISP = "/root/reference/fieldexamples/outcrop/outcropDisp.mat"
+
+
+def test_norm_trace_closed_form():
+    r = np.random.default_rng(1)
+    d = r.standard_normal((50, 7))
+    gains = 10.0 ** r.uniform(-3, 3, 7)
+    out = O.norm_trace(d * gains)
+    assert np.allclose(np.max(np.abs(out), axis=0), 1.0, rtol=0, atol=1e-15)
+    assert np.array_equal(np.sign(out), np.sign(d))
+    assert np.allclose(out, O.norm_trace(d), rtol=1e-13, atol=0)       # positive column gains drop out
+    k = np.argmax(np.abs(d[:, 2]))
+    assert out[k, 2] == np.sign(d[k, 2])
+
+
+def test_norm_trace_nan_and_zero_columns():
+    d = np.array([[1.0, 0.0, np.nan], [-4.0, 0.0, 2.0], [np.nan, 0.0, -8.0]])
+    out = O.norm_trace(d)
+    assert np.allclose(out[:2, 0], [0.25, -1.0]) and np.isnan(out[2, 0])   # max skips NaN
+    assert np.all(np.isnan(out[:, 1]))                                     # 0/0
+    assert np.isnan(out[0, 2]) and np.allclose(out[1:, 2], [0.25, -1.0])
+
+
+def _spectrum(np_=40, npair=14, strong=9, seed=0):
+    """Unit-peak Gaussian columns along a smooth ridge, |mm| column gains
+    with the largest at column ``strong`` (0-based)."""
+    r = np.random.default_rng(seed)
+    j = np.arange(1, np_ + 1)[:, None]
+    ridge = np.round(12 + 10 * np.linspace(0, 1, npair) ** 1.5).astype(int)
+    ml = np.exp(-0.5 * ((j - ridge[None, :]) / 2.5) ** 2) + 0.01 * r.random((np_, npair))
+    ml = ml / ml.max(axis=0, keepdims=True)
+    gain = np.linspace(1.0, 3.0, npair)
+    gain[strong] = 10.0
+    return ml, ml * gain[None, :], ridge
+
+
+def test_field_pick_start_and_window():
+    ml, mlr, ridge = _spectrum()
+    cr0 = np.zeros(ml.shape[1])
+    res, cr, ini = O.LHDispPick_field(ml, mlr, 100.0, cr0, 1, 0, "ml")
+    assert ini == 10                                                   # 1-based column of the largest |mm|
+    assert np.array_equal(cr - 100.0, ridge)                           # the ridge, picked from there
+    res2, cr2, _ = O.LHDispPick_field(ml, mlr, 100.0, cr0, 1, 0, "mlr")
+    assert np.array_equal(cr, cr2)                                     # pickFDCValue from the argmax row
+    assert np.array_equal(O.pickFDC1(ml, 10, 2, 4), O.pickFDC1(ml, 10, 2, 4, lpvalue=ridge[9]))
+    d = np.arange(ml.shape[1], dtype=float) - 3.0
+    res3, _, _ = O.LHDispPick_field(ml, mlr, 100.0, cr - d, 3, 4, "ml")
+    assert res3 == pytest.approx(np.mean(d[2:ml.shape[1] - 4] ** 2), rel=1e-15)
+    assert res == pytest.approx(np.mean(cr ** 2))
+    # the misfit window of SWD.m's picker path through pick_side
+    r4, c4 = O.pick_side(ml, {"ml1": mlr}, 100.0, cr - d, 10, {"res_first": 3, "res_last": 4})
+    assert np.array_equal(c4, cr) and r4 == pytest.approx(res3)
+
+
+def test_adwdgrad_peak_norm_closed_form():
+    r = np.random.default_rng(5)
+    nt, ng, np_, L = 64, 9, 30, 6
+    seismo = r.standard_normal((nt, ng))
+    radon = dict(df=10.0, dt=1e-3, np_=np_, vmin=100.0, vmax=100.0 + np_ - 1, fmin=100.0, fmax=300.0)
+    ml, _, sfb = O.rt_side(seismo, "r", 1, radon["df"], radon["dt"], np_, radon["vmin"], radon["vmax"],
+                           radon["fmin"], radon["fmax"], nt, ng, 0.5, L, 0, 1)
+    g = np.round(r.normal(0, 3, ml.shape[1]))
+    a = O.ADWDgrad_1(nt, ng, 5, ml.shape[1], 1, 3, 0, 1, 10.0, g, g, sfb, None)        # right branch only
+    an = O.ADWDgrad_1(nt, ng, 5, ml.shape[1], 1, 3, 0, 1, 10.0, g, g, sfb, None, peak_norm=True)
+    assert np.max(np.abs(an)) == pytest.approx(1.0, abs=1e-15)
+    assert np.allclose(an, a / np.max(np.abs(a)), rtol=1e-15, atol=0)
+
+
+def test_variant_structs():
+    v = F.variant_struct("outcrop")
+    assert (v.norm_trace, v.auto_start, v.res_first, v.res_last, v.adj_peak_norm) == (1, 1, 15, 15, 1)
+    assert v.vp_ratio == pytest.approx(1.732)
+    q = F.variant_struct("qadimah")
+    assert (q.res_first, q.res_last) == (50, 30)
+    z = F.variant_struct("swd")
+    assert (z.norm_trace, z.auto_start, z.res_first, z.res_last, z.adj_peak_norm, z.vp_ratio) == (0, 0, 0, 0, 0, 0)
+    with pytest.raises(ValueError):
+        F.variant_struct({"norm": 1})
+
+
+def test_manual_pick_slicing_rules():
+    nf, ns, no = 991, 4, 20
+    cube = np.arange(nf * ns * no, dtype=float).reshape((nf, ns, no), order="F")
+    cr = F.manual_picks(cube, 20.0, 85.0, 0.1, 50, 5)          # outcrop.m:148-157: rows 191:841, offset 10
+    assert cr.shape == (651, ns)
+    assert np.array_equal(cr, cube[190:841, :, 9])
+    cr = F.manual_picks(cube, 10.0, 60.0, 0.1, 39, 3)          # QademahSWD3.m:149-153: rows 91:591, offset 13
+    assert cr.shape == (501, ns)
+    assert np.array_equal(cr, cube[90:591, :, 12])
+    with pytest.raises(ValueError):
+        F.manual_picks(cube, 20.0, 85.0, 0.1, 52, 5)           # 52/5 is not an offset index
+    with pytest.raises(ValueError):
+        F.manual_picks(cube, 20.0, 120.0, 0.1, 50, 5)          # past the cube's 100 Hz
+
+
+def test_outcrop_geometry():
+    prob, mod = F.outcrop()
+    assert prob["nt"] == 7450 and prob["nt"] % prob["in_wf"] == 0
+    vpmax = 671 * 1.6 * 1.732
+    assert vpmax * prob["dt"] / prob["dx"] == pytest.approx(0.5)
+    assert np.size(prob["sx"]) == 60 and prob["ng"] == 60 and prob["M"] == 1
+    assert prob["fmin"] == 20.0 and F.outcrop_fmin(5) == 25.0
+    from swd_amd.engine import problem_npair
+    assert problem_npair(prob) == 651
+    assert mod["vp"][0, 0] == pytest.approx(150 * 1.732)
+
+
+@pytest.mark.skipif(not os.path.exists(OUTCROP_DISP), reason="the outcrop pick cube is not present here")
+def test_outcrop_manual_cube_fits_the_outcrop_problem():
+    crr, crl = F.load_manual_cubes(OUTCROP_DISP)
+    assert crr.shape == (991, 60, 10) and crl.shape == (991, 60, 10)
+    prob, _ = F.outcrop(offset=50)
+    from swd_amd.engine import problem_npair
+    for cube in (crr, crl):
+        cr = F.manual_picks(cube, prob["fmin"], prob["fmax"], prob["df"], prob["offset"], 5)
+        assert cr.shape == (problem_npair(prob), 60)
+        assert np.all(np.isfinite(cr)) and cr.min() >= 1 and cr.max() <= 1000
