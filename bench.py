@@ -266,6 +266,12 @@ def dispersion_leg(betas=(5.0, 10.0, 20.0), reps=3):
         adj16 = min(D.adjoint(10.0, want=False)[1] for _ in range(reps))
     finally:
         set_radon_precision(old)
+    # the field-script variant with the QadimahFault FK == 1 branch (fk_filter per window) and norm_trace
+    D.set_variant({"fk_filter": 1, "norm_trace": 1})
+    D.pick()
+    pick_fk = min(D.pick()["ms"] for _ in range(reps))
+    D.adjoint(10.0, want=False)
+    adj_fk = min(D.adjoint(10.0, want=False)[1] for _ in range(reps))
     D.close()
     same_bin = float(np.mean(np.abs(r16["cr_r"] - r["cr_r"]) <= 1.0))
     nt, ng, npair, S = prob["nt"], prob["ng"], D.npair, 60
@@ -292,6 +298,9 @@ def dispersion_leg(betas=(5.0, 10.0, 20.0), reps=3):
                        "right_picks_within_1_bin_of_fp32": round(same_bin, 4),
                        "what": "DFT and inverse-DFT GEMMs in BF16 split precision (bf16 hi + lo, 3 MFMA products); "
                                "slant stacks stay FP32"},
+            "fk_norm": {"pick_ms": round(pick_fk, 3), "adjoint_ms_beta10": round(adj_fk, 3),
+                        "what": "norm_trace + fk_filter on every window (one more DFT-sized GEMM and the window "
+                                "Hilbert mix)"},
             "picks_within_1_bin_of_c_15_70hz": round(float(np.mean(np.concatenate(ok))), 4),
             "pick_columns_judged": len(ok),
             "what": "times include the small kernels of each stage (gather, |mm|, picks, softargmax, placement)"}

@@ -214,11 +214,17 @@ int swd_engine_set_fd_path(swd_engine *e, int mode);
  *                  vp = vp_ratio * vs1 (vp = vs1*1.732, outcrop.m:217,245,271,
  *                  301,331); 0 keeps vp fixed (SWD.m).  The dispersion batch
  *                  ignores it.
+ *   fk_filter      the QadimahFault RTr/RTl branch FK == 1 (RTr.m:14-16,
+ *                  RTl.m:15-17; core/fk_filter.m:48-53,283-296): each window,
+ *                  flipped for the left side, passes the down-dip f-k filter
+ *                  before its spectrum (no derivative through it, as there).
+ *                  Windows of at most 2048 traces.
  * The variant applies to every later dispersion pass of the handle (observed
  * picks included). */
 typedef struct swd_variant {
     int norm_trace, auto_start, res_first, res_last, adj_peak_norm;
     double vp_ratio;
+    int fk_filter;
 } swd_variant;
 int swd_engine_set_variant(swd_engine *e, const swd_variant *v);
 /* One stage of swd_engine_gradient on the resident buffers: 1 forward FD +

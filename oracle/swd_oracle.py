@@ -53,6 +53,8 @@ PARITY = {
                   "column gains, NaN samples skipped by the max, a zero column gives NaN",
     "LHDispPick_field": "hand-built spectra whose largest |mm| sits in a known column (start column and row), "
                         "closed-form windowed misfit, pickFDCValue == pickFDC1 when the given row is the argmax",
+    "fk_filter": "brute-force 2-D DFT restatement of the MATLAB loops on tiny inputs (explicit sums, explicit "
+                 "shift indices), real output, linearity",
     "ADWDgrad_1 peak_norm": "closed form: peak |a| becomes 1, the result is the unnormalised source over its peak",
     "swd_line_search": "quadratic misfit in the step factor: the parabola step lands on the exact minimum; "
                        "the halving branch, the doubling branch, the fallback to the better trial and the "
@@ -269,13 +271,34 @@ def rt_window(side: str, is_: int, b: int, offset: int, m: int, M: int) -> Tuple
     return nend + 1, mround(src - m)
 
 
-def rt_side(seismo, side: str, is_, df, dt, np_, vmin, vmax, fmin, fmax, a, b, ds, offset, m, M):
-    """RTr/RTl (+AD) in float64.  Returns (ml, dataLen, saveForBackward dict)."""
+def fk_filter(d, dt=None, dx=None) -> np.ndarray:
+    """fieldexamples/QadimahFault/core/fk_filter.m:48-53,283-296 (the branch
+    that runs: the down-dip case): zero-pad d (ns x ntr) to 2ns x 2ntr,
+    fft2, fftshift, zero the shifted quadrants (1:ns, 1:ntr) and
+    (ns+1:2ns, ntr+1:2ntr), shift back, real(ifft2), crop to ns x ntr.
+    dt and dx only label the axes there and do not enter."""
+    d = np.asarray(d, dtype=np.float64)
+    ns, ntr = d.shape
+    D = np.zeros((2 * ns, 2 * ntr))
+    D[:ns, :ntr] = d
+    D = np.fft.fftshift(np.fft.fft2(D))
+    D[:ns, :ntr] = 0
+    D[ns:, ntr:] = 0
+    return np.real(np.fft.ifft2(np.fft.fftshift(D)))[:ns, :ntr]
+
+
+def rt_side(seismo, side: str, is_, df, dt, np_, vmin, vmax, fmin, fmax, a, b, ds, offset, m, M, fk: bool = False):
+    """RTr/RTl (+AD) in float64.  Returns (ml, dataLen, saveForBackward dict).
+    ``fk`` is the QadimahFault copies' FK == 1 branch: fk_filter on the
+    (flipped) window before the FFT (fieldexamples/QadimahFault/core/RTr.m:14-16,
+    RTl.m:15-17)."""
     seismo = np.asarray(seismo, dtype=np.float64)
     first, last = rt_window(side, is_, b, offset, m, M)
     uxt = seismo[:, first - 1:last]
     if side == "l":
         uxt = uxt[:, ::-1]
+    if fk:
+        uxt = fk_filter(uxt, dt, ds)
     L = uxt.shape[1]
     x = (m + np.linspace(0, L - 1, L)) * ds
     ccn = ccn_of(df, dt)
@@ -703,18 +726,19 @@ def swd_shot(vp, vs, s, nbc, nt, dx, dt, sx, sz, gx, gz, in_wf, fd_order, is_, n
     np_ = r["vmax"] - r["vmin"] + 1
     out = {"seismo_w": sw}
     sw_rt = norm_trace(sw) if v.get("norm_trace") else sw
+    fk = bool(v.get("fk_filter"))
     gr = gl = np.zeros(mround((r["fmax"] - r["fmin"]) / r["df"]) + 1)
     sfbr = sfbl = None
     M, m = r["M"], r["m"]
     if is_ <= ns - math.floor(m / M) - mround(w / M):
         ml, _, sfbr = rt_side(sw_rt, "r", is_, r["df"], dt, np_, r["vmin"], r["vmax"], r["fmin"], r["fmax"], nt, ng,
-                              r["ds"], r["offset"], m, M)
+                              r["ds"], r["offset"], m, M, fk)
         out["res_r"], out["cr_r"] = pick_side(ml, sfbr, r["vmin"], cr0_r, ini, v)
         out["ml_r"] = ml
         gr = out["cr_r"] - np.ravel(cr0_r)
     if is_ >= mround(w / M) + math.floor(m / M) + 1:
         ml, _, sfbl = rt_side(sw_rt, "l", is_, r["df"], dt, np_, r["vmin"], r["vmax"], r["fmin"], r["fmax"], nt, ng,
-                              r["ds"], r["offset"], m, M)
+                              r["ds"], r["offset"], m, M, fk)
         out["res_l"], out["cr_l"] = pick_side(ml, sfbl, r["vmin"], cr0_l, ini, v)
         out["ml_l"] = ml
         gl = out["cr_l"] - np.ravel(cr0_l)

@@ -189,32 +189,210 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GemmArgs g)
 }
 
 // ---------------------------------------------------------------------------
-// Trace peaks of norm_trace (fieldexamples/outcrop/core/norm_trace.m:8-16):
-// peak[shot][g] = max_t |traces[shot][t][g]| over all nt samples.  fmaxf
-// skips NaN samples as MATLAB's max does.  64 traces per block (one per
-// lane, coalesced rows of g), the four waves split t and meet in LDS.
+// Per-trace statistics in one pass over the gathers traces[shot][t][g]:
+//   peak[shot][g]     = max_t |x| over all nt samples (norm_trace's divisor,
+//                       fieldexamples/outcrop/core/norm_trace.m:8-16; fmaxf
+//                       skips NaN samples as MATLAB's max does);
+//   par[shot][g][0/1] = sum of x over even / odd t (FP64; the fk-filter
+//                       spectrum below needs them).
+// Either output may be null.  64 traces per block (one per lane, coalesced
+// rows of g); wave w of 4 takes t = w, w + 4, ..., so waves 0, 2 see even t
+// and waves 1, 3 odd t; the four meet in LDS in a fixed order.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void trace_peak_kernel(const float* __restrict__ traces, int nt, int ng,
-                                                         float* __restrict__ peak)
+__global__ __launch_bounds__(256) void trace_stats_kernel(const float* __restrict__ traces, int nt, int ng,
+                                                          float* __restrict__ peak, double* __restrict__ par)
 {
     __shared__ float red[4][64];
+    __shared__ double rs[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int g = blockIdx.x * 64 + lane;
     const int shot = blockIdx.y;
     float m0 = 0.f, m1 = 0.f;
+    double s0 = 0.0, s1 = 0.0;
     if (g < ng) {
         const float* p = traces + (size_t)shot * nt * ng + g;
         int t = w;
         for (; t + 4 < nt; t += 8) {
-            m0 = fmaxf(m0, fabsf(p[(size_t)t * ng]));
-            m1 = fmaxf(m1, fabsf(p[(size_t)(t + 4) * ng]));
+            const float a = p[(size_t)t * ng], b = p[(size_t)(t + 4) * ng];
+            m0 = fmaxf(m0, fabsf(a));
+            m1 = fmaxf(m1, fabsf(b));
+            s0 += (double)a;
+            s1 += (double)b;
         }
-        if (t < nt) m0 = fmaxf(m0, fabsf(p[(size_t)t * ng]));
+        if (t < nt) {
+            const float a = p[(size_t)t * ng];
+            m0 = fmaxf(m0, fabsf(a));
+            s0 += (double)a;
+        }
     }
     red[w][lane] = fmaxf(m0, m1);
+    rs[w][lane] = s0 + s1;
     __syncthreads();
-    if (w == 0 && g < ng)
-        peak[(size_t)shot * ng + g] = fmaxf(fmaxf(red[0][lane], red[1][lane]), fmaxf(red[2][lane], red[3][lane]));
+    if (w == 0 && g < ng) {
+        const size_t o = (size_t)shot * ng + g;
+        if (peak) peak[o] = fmaxf(fmaxf(red[0][lane], red[1][lane]), fmaxf(red[2][lane], red[3][lane]));
+        if (par) {
+            par[2 * o] = rs[0][lane] + rs[2][lane];
+            par[2 * o + 1] = rs[1][lane] + rs[3][lane];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fk_filter (fieldexamples/QadimahFault/core/fk_filter.m:48-53,283-296, run by
+// RTr/RTl when FK == 1 on each window X (nt x L) before the FFT) folded into
+// the spectrum at the bins.  The filter keeps the (f < 0, k >= 0) and
+// (f >= 0, k < 0) quadrants of the 2-D DFT of X zero-padded to N x M
+// (N = 2 nt, M = 2 L), so it is real-linear and separable into one-sided
+// filters along t and x.  With W the DFT at the bins (t < nt_eff), the
+// spectrum of the filtered window is, column l of the window:
+//   Df[:, l] = D[:, l]/2 - (1/M) sum_{l - l' odd} [ (2/N)(O Xe[l'] + E Xo[l'])
+//                                                 + 2 cot(pi (l - l')/M) H[:, l'] ]
+// where D = W^T X, E / O = sums of W over even / odd t, Xe / Xo = sums of each
+// trace over even / odd t, and H = U' X with
+//   U'[k2][s] = -(1/N) sum_{t < nt_eff, t - s odd} W[t][k2] cot(pi (t - s)/N)
+// (the discrete Hilbert kernel of the one-sided time filter).  D and H are
+// two GEMMs over the whole gathers; the window kernel below mixes each
+// window's traces.
+// ---------------------------------------------------------------------------
+
+// cotab[tau] = cot(pi tau / N) for tau = 1 .. n-1 (FP64); cotab[0] = 0
+__global__ void cot_table_kernel(double* cotab, int n, double N)
+{
+    const int tau = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tau >= n) return;
+    if (tau == 0) { cotab[0] = 0.0; return; }
+    double sv, cv;
+    sincospi((double)tau / N, &sv, &cv);
+    cotab[tau] = cv / sv;
+}
+
+// U'[s][k2] (layout of W: [s][2 npair]) in FP64 from W[t][k2], s < nt.
+__global__ __launch_bounds__(64) void fk_hilbert_table_kernel(const float* __restrict__ W, int nt_eff, int nt,
+                                                              int npair2, const double* __restrict__ cotab,
+                                                              float* __restrict__ Up)
+{
+    const int k2 = blockIdx.x * 64 + threadIdx.x;
+    const int s = blockIdx.y;
+    if (k2 >= npair2) return;
+    double acc = 0.0;
+    for (int t = (s + 1) & 1; t < nt_eff; t += 2) {
+        const int tau = t - s;
+        const double c = tau > 0 ? cotab[tau] : -cotab[-tau];
+        acc += (double)W[(size_t)t * npair2 + k2] * c;
+    }
+    Up[(size_t)s * npair2 + k2] = (float)(-acc / (2.0 * nt));
+}
+
+// EO[0][k2] = sum_{t even < nt_eff} W[t][k2], EO[1][k2] = the odd-t sum (FP64)
+__global__ void fk_parity_table_kernel(const float* __restrict__ W, int nt_eff, int npair2, double* __restrict__ EO)
+{
+    const int k2 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k2 >= npair2) return;
+    double e = 0.0, o = 0.0;
+    for (int t = 0; t < nt_eff; t += 2) e += (double)W[(size_t)t * npair2 + k2];
+    for (int t = 1; t < nt_eff; t += 2) o += (double)W[(size_t)t * npair2 + k2];
+    EO[k2] = e;
+    EO[npair2 + k2] = o;
+}
+
+constexpr int FKK = 8;      // frequencies per block (16 rows of the real representation)
+constexpr int FKL = 256;    // window positions per block and per staged chunk
+constexpr int kFkMaxL = 2048;
+
+// X[k][l][col] = filtered spectrum of window col at frequency k, position l
+// (0 beyond L, as the plain gather).  With trace peaks, every trace enters
+// divided by its peak (norm_trace before the window).  Block: one col, FKK
+// frequencies, FKL positions; the col's H rows are staged through LDS in
+// chunks of FKL positions, the cot weights of the window in LDS.
+__global__ __launch_bounds__(FKL) void fk_window_kernel(const float* __restrict__ D, const float* __restrict__ H,
+                                                        const double* __restrict__ par, const float* __restrict__ peak,
+                                                        const double* __restrict__ EO, const DispCol* __restrict__ cols,
+                                                        int ncol, int npair, int ng, int Lmax, int nt, float2* X)
+{
+    __shared__ float Hs[2 * FKK][FKL];
+    __shared__ float ctab[kFkMaxL];
+    __shared__ double red[4][FKL / 64];
+    const int col = blockIdx.x;
+    const int k0 = blockIdx.y * FKK;
+    const int tid = threadIdx.x;
+    const int l = blockIdx.z * FKL + tid;
+    const DispCol c = cols[col];
+    const int npair2 = 2 * npair;
+    const int L = c.enabled ? c.L : 0;
+    if (L == 0 || blockIdx.z * FKL >= L) {
+        if (l < Lmax)
+            for (int kk = 0; kk < FKK && k0 + kk < npair; ++kk)
+                X[((size_t)(k0 + kk) * Lmax + l) * ncol + col] = make_float2(0.f, 0.f);
+        return;  // uniform over the block
+    }
+    const double M = 2.0 * L, N = 2.0 * nt;
+    const float* Dc = D + (size_t)c.shot * npair2 * ng;
+    const float* Hc = H + (size_t)c.shot * npair2 * ng;
+    auto qof = [&](int lp) -> float { return peak ? 1.0f / peak[(size_t)c.shot * ng + c.tr0 + c.dir * lp] : 1.0f; };
+    // cot(pi tau / M), tau = 0 .. L-1, and the parity-class sums of the (scaled) trace sums
+    for (int tau = tid; tau < L; tau += FKL) {
+        double sv, cv;
+        sincospi((double)tau / M, &sv, &cv);
+        ctab[tau] = tau ? (float)(cv / sv) : 0.f;
+    }
+    double se0 = 0.0, se1 = 0.0, so0 = 0.0, so1 = 0.0;
+    for (int lp = tid; lp < L; lp += FKL) {
+        const size_t o = 2 * ((size_t)c.shot * ng + c.tr0 + c.dir * lp);
+        const double q = (double)qof(lp);
+        const double xe = par[o] * q, xo = par[o + 1] * q;
+        if (lp & 1) { se1 += xe; so1 += xo; } else { se0 += xe; so0 += xo; }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        se0 += __shfl_xor(se0, off); se1 += __shfl_xor(se1, off);
+        so0 += __shfl_xor(so0, off); so1 += __shfl_xor(so1, off);
+    }
+    if ((tid & 63) == 0) { red[0][tid >> 6] = se0; red[1][tid >> 6] = se1; red[2][tid >> 6] = so0; red[3][tid >> 6] = so1; }
+    float acc[2 * FKK];
+#pragma unroll
+    for (int r = 0; r < 2 * FKK; ++r) acc[r] = 0.f;
+    const int par_l = l & 1;
+    for (int c0 = 0; c0 < L; c0 += FKL) {
+        __syncthreads();
+        for (int e = tid; e < 2 * FKK * FKL; e += FKL) {
+            const int r = e / FKL, j = e % FKL;
+            const int k2 = r < FKK ? k0 + r : npair + k0 + (r - FKK);
+            const int lp = c0 + j;
+            float v = 0.f;
+            if (lp < L && k0 + (r % FKK) < npair) v = Hc[(size_t)k2 * ng + c.tr0 + c.dir * lp] * qof(lp);
+            Hs[r][j] = v;
+        }
+        __syncthreads();
+        if (l < L) {
+            const int jend = min(FKL, L - c0);
+            for (int j = (par_l + 1 + c0) & 1; j < jend; j += 2) {
+                const int tau = l - (c0 + j);
+                const float cw = tau > 0 ? ctab[tau] : -ctab[-tau];
+#pragma unroll
+                for (int r = 0; r < 2 * FKK; ++r) acc[r] = fmaf(cw, Hs[r][j], acc[r]);
+            }
+        }
+    }
+    if (l >= Lmax) return;
+    if (l >= L) {
+        for (int kk = 0; kk < FKK && k0 + kk < npair; ++kk)
+            X[((size_t)(k0 + kk) * Lmax + l) * ncol + col] = make_float2(0.f, 0.f);
+        return;
+    }
+    // sums over the parity class opposite to l
+    const double se = par_l ? red[0][0] + red[0][1] + red[0][2] + red[0][3] : red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    const double so = par_l ? red[2][0] + red[2][1] + red[2][2] + red[2][3] : red[3][0] + red[3][1] + red[3][2] + red[3][3];
+    const int g = c.tr0 + c.dir * l;
+    const float ql = qof(l);
+    const double cc = 2.0 / (M * N);
+    for (int kk = 0; kk < FKK && k0 + kk < npair; ++kk) {
+        const int kr = k0 + kk, ki = npair + k0 + kk;
+        const double re = 0.5 * (double)(Dc[(size_t)kr * ng + g] * ql) - cc * (EO[npair2 + kr] * se + EO[kr] * so) -
+                          (2.0 / M) * (double)acc[kk];
+        const double im = 0.5 * (double)(Dc[(size_t)ki * ng + g] * ql) - cc * (EO[npair2 + ki] * se + EO[ki] * so) -
+                          (2.0 / M) * (double)acc[FKK + kk];
+        X[((size_t)kr * Lmax + l) * ncol + col] = make_float2((float)re, (float)im);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -713,10 +891,33 @@ int launch_shot_peak_norm(hipStream_t st, float* a, size_t n, int nshots, float*
     return kOk;
 }
 
-int launch_trace_peak(hipStream_t st, const float* traces, int nt, int ng, int nshots, float* peak)
+int launch_trace_stats(hipStream_t st, const float* traces, int nt, int ng, int nshots, float* peak, double* par)
 {
-    if (nt <= 0 || ng <= 0 || nshots <= 0) return kOk;
-    trace_peak_kernel<<<dim3(div_up(ng, 64), nshots), 256, 0, st>>>(traces, nt, ng, peak);
+    if (nt <= 0 || ng <= 0 || nshots <= 0 || (!peak && !par)) return kOk;
+    trace_stats_kernel<<<dim3(div_up(ng, 64), nshots), 256, 0, st>>>(traces, nt, ng, peak, par);
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
+int launch_fk_tables(hipStream_t st, const float* W, int nt_eff, int nt, int npair, double* cotab, float* Up,
+                     double* EO)
+{
+    const int npair2 = 2 * npair;
+    cot_table_kernel<<<div_up(nt, 256), 256, 0, st>>>(cotab, nt, 2.0 * nt);
+    SWD_HIP(hipGetLastError());
+    fk_hilbert_table_kernel<<<dim3(div_up(npair2, 64), nt), 64, 0, st>>>(W, nt_eff, nt, npair2, cotab, Up);
+    SWD_HIP(hipGetLastError());
+    fk_parity_table_kernel<<<div_up(npair2, 64), 64, 0, st>>>(W, nt_eff, npair2, EO);
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
+int launch_fk_window(hipStream_t st, const float* D, const float* H, const double* par, const float* peak,
+                     const double* EO, const DispCol* cols, int ncol, int npair, int ng, int Lmax, int nt, float2* X)
+{
+    if (Lmax > kFkMaxL) { set_error("fk filter windows of %d traces exceed %d", Lmax, kFkMaxL); return kErrArg; }
+    const dim3 grid(ncol, div_up(npair, FKK), div_up(Lmax, FKL));
+    fk_window_kernel<<<grid, FKL, 0, st>>>(D, H, par, peak, EO, cols, ncol, npair, ng, Lmax, nt, X);
     SWD_HIP(hipGetLastError());
     return kOk;
 }

@@ -47,8 +47,17 @@ int set_radon_precision(int mode);
 // part holds nshots x kPeakParts partial maxima
 constexpr int kPeakParts = 64;
 int launch_shot_peak_norm(hipStream_t st, float* a, size_t n, int nshots, float* part);
-// peak[shot][g] = max_t |traces[shot][t][g]| (norm_trace's divisor)
-int launch_trace_peak(hipStream_t st, const float* traces, int nt, int ng, int nshots, float* peak);
+// peak[shot][g] = max_t |traces[shot][t][g]| (norm_trace's divisor) and
+// par[shot][g][0/1] = even / odd-t sums (fk filter); either may be null
+int launch_trace_stats(hipStream_t st, const float* traces, int nt, int ng, int nshots, float* peak, double* par);
+// fk filter tables from W (nt rows): cotab (nt doubles, scratch), U' ([s][2 npair]
+// like W, s < nt) and EO (even / odd-t sums of W, 2 x 2 npair doubles)
+int launch_fk_tables(hipStream_t st, const float* W, int nt_eff, int nt, int npair, double* cotab, float* Up,
+                     double* EO);
+// window spectra of fk-filtered windows (X as launch_radon_gather writes it)
+// from D = W^T traces, H = U'^T traces and the trace sums; peak may be null
+int launch_fk_window(hipStream_t st, const float* D, const float* H, const double* par, const float* peak,
+                     const double* EO, const DispCol* cols, int ncol, int npair, int ng, int Lmax, int nt, float2* X);
 // peak may be null (no trace normalisation)
 int launch_radon_gather(hipStream_t st, const float* D, const DispCol* cols, int ncol, int npair, int ng, int Lmax,
                         const float* peak, float2* X);

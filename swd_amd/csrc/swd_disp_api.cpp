@@ -111,7 +111,10 @@ int DispPlan::init(const DispGeom& geom, const std::vector<DispCol>& columns, hi
 int DispPlan::forward(const float* traces)
 {
     int rc;
-    if (norm_traces && (rc = launch_trace_peak(st, traces, g.nt, g.ng, g.nshots, peak.as<float>()))) return rc;
+    if ((norm_traces || fk) &&
+        (rc = launch_trace_stats(st, traces, g.nt, g.ng, g.nshots, norm_traces ? peak.as<float>() : nullptr,
+                                 fk ? fkPar.as<double>() : nullptr)))
+        return rc;
     GemmArgs a;
     memset(&a, 0, sizeof a);
     a.A = W.as<float>();                 // A(m = k2, k = t) = W[t][k2]
@@ -122,15 +125,42 @@ int DispPlan::forward(const float* traces)
     a.sCm = g.ng; a.sCn = 1; a.sCb = 2LL * g.npair * g.ng;
     a.M = 2 * g.npair; a.N = g.ng; a.K = g.nt_eff; a.alpha = 1.f;
     if ((rc = launch_gemm_f32(st, a, g.nshots))) return rc;
-    if ((rc = launch_radon_gather(st, D.as<float>(), dcols.as<DispCol>(), ncol, g.npair, g.ng, Lmax,
-                                  norm_traces ? peak.as<float>() : nullptr, X.as<float2>())))
+    if (fk) {
+        // H = U'^T traces over all nt samples, then the windows' filtered spectra
+        a.A = fkUp.as<float>();
+        a.C = fkH.as<float>();
+        a.K = g.nt;
+        if ((rc = launch_gemm_f32(st, a, g.nshots))) return rc;
+        if ((rc = launch_fk_window(st, D.as<float>(), fkH.as<float>(), fkPar.as<double>(),
+                                   norm_traces ? peak.as<float>() : nullptr, fkEO.as<double>(), dcols.as<DispCol>(),
+                                   ncol, g.npair, g.ng, Lmax, g.nt, X.as<float2>())))
+            return rc;
+    } else if ((rc = launch_radon_gather(st, D.as<float>(), dcols.as<DispCol>(), ncol, g.npair, g.ng, Lmax,
+                                         norm_traces ? peak.as<float>() : nullptr, X.as<float2>()))) {
         return rc;
+    }
     SlantArgs s;
     s.in = X.as<float2>(); s.out = MM.as<float2>();
     s.np = g.np; s.Lmax = Lmax; s.ncol = ncol; s.npair = g.npair; s.c0 = g.c0; s.m = g.m;
     s.df = g.df; s.vmin = g.vmin; s.ds = g.ds;
     if ((rc = launch_slant_stack(st, s, false))) return rc;
     return launch_radon_abs(st, MM.as<float2>(), g.np, g.npair, ncol, ml.as<float>(), mlr.as<float>());
+}
+
+int DispPlan::enable_fk()
+{
+    int rc;
+    const size_t npair2 = 2 * (size_t)g.npair;
+    if ((rc = fkUp.alloc((size_t)g.nt * npair2 * sizeof(float))) ||
+        (rc = fkH.alloc((size_t)g.nshots * npair2 * g.ng * sizeof(float))) ||
+        (rc = fkPar.alloc((size_t)g.nshots * g.ng * 2 * sizeof(double))) ||
+        (rc = fkEO.alloc(2 * npair2 * sizeof(double))) || (rc = fkCot.alloc((size_t)g.nt * sizeof(double))))
+        return rc;
+    if ((rc = launch_fk_tables(st, W.as<float>(), g.nt_eff, g.nt, g.npair, fkCot.as<double>(), fkUp.as<float>(),
+                               fkEO.as<double>())))
+        return rc;
+    SWD_HIP(hipStreamSynchronize(st));
+    return kOk;
 }
 
 int DispPlan::pick(int lp, float fg, float lfg)

@@ -47,6 +47,15 @@ struct DispPlan {
     bool auto_start = false;     // pick from the column of the largest |mm|
     int res_first = 1, res_last = 0;  // misfit over cr(res_first : end - res_last), 1-based
     bool adj_peak_norm = false;  // divide each shot's adjoint source by its peak |a|
+    bool fk = false;             // fk_filter each window before its spectrum
+    DevBuf fkUp;     // [t][2 npair] Hilbert table U' of the fk filter
+    DevBuf fkH;      // [shot][2 npair][g] U'^T traces
+    DevBuf fkPar;    // [shot][g][2] even / odd-t trace sums (double)
+    DevBuf fkEO;     // [2][2 npair] even / odd-t sums of W (double)
+    DevBuf fkCot;    // [nt] cot(pi tau / 2nt) (double, table scratch)
+
+    // allocate and fill the fk filter tables (once)
+    int enable_fk();
 
     int init(const DispGeom& geom, const std::vector<DispCol>& columns, hipStream_t stream);
     int forward(const float* traces);             // traces [shot][t][g]
@@ -54,13 +63,19 @@ struct DispPlan {
     int adjoint(double beta, float* adj_out);     // adj_out [shot][t][g]
 };
 
-inline void set_plan_variant(DispPlan& d, const swd_variant& v)
+inline int set_plan_variant(DispPlan& d, const swd_variant& v)
 {
+    if (v.fk_filter && !d.fkUp.p) {
+        const int rc = d.enable_fk();
+        if (rc) return rc;
+    }
     d.norm_traces = v.norm_trace != 0;
     d.auto_start = v.auto_start != 0;
     d.res_first = v.res_first > 1 ? v.res_first : 1;
     d.res_last = v.res_last > 0 ? v.res_last : 0;
     d.adj_peak_norm = v.adj_peak_norm != 0;
+    d.fk = v.fk_filter != 0;
+    return 0;
 }
 
 }  // namespace swd

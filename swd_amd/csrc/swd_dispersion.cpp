@@ -130,7 +130,8 @@ int swd_dispersion_set_variant(swd_dispersion* h, const swd_variant* v)
         set_error("bad misfit window cr(%d:end-%d) for %d columns", v->res_first, v->res_last, d.disp.g.npair);
         return kErrArg;
     }
-    set_plan_variant(d.disp, *v);
+    int rc;
+    if ((rc = set_plan_variant(d.disp, *v))) return rc;
     d.have_picks = false;
     return kOk;
 }

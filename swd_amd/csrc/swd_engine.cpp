@@ -585,7 +585,8 @@ int swd_engine_set_variant(swd_engine* e, const swd_variant* v)
         set_error("misfit window cr(%d:end-%d) is empty for %d columns", v->res_first, v->res_last, g.npair);
         return kErrArg;
     }
-    set_plan_variant(g.disp, *v);
+    int rc;
+    if ((rc = set_plan_variant(g.disp, *v))) return rc;
     g.vp_ratio = v->vp_ratio;
     return kOk;
 }

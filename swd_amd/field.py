@@ -23,7 +23,8 @@ from .fd import ricker
 
 class Variant(ctypes.Structure):
     _fields_ = [("norm_trace", ctypes.c_int), ("auto_start", ctypes.c_int), ("res_first", ctypes.c_int),
-                ("res_last", ctypes.c_int), ("adj_peak_norm", ctypes.c_int), ("vp_ratio", ctypes.c_double)]
+                ("res_last", ctypes.c_int), ("adj_peak_norm", ctypes.c_int), ("vp_ratio", ctypes.c_double),
+                ("fk_filter", ctypes.c_int)]
 
 
 VARIANTS = {
@@ -34,7 +35,7 @@ VARIANTS = {
     # normalisation, vp = vs1*1.732 (outcrop.m:217)
     "outcrop": dict(norm_trace=1, auto_start=1, res_first=15, res_last=15, adj_peak_norm=1, vp_ratio=1.732),
     # fieldexamples/QadimahFault: as the outcrop with the misfit over cr_pre(50:end-30)
-    # (QadimahFault/core/LHDispPick.m:25)
+    # (QadimahFault/core/LHDispPick.m:25); FK = 0 there (QademahSWD3.m:119), fk_filter=1 is its FK = 1
     "qadimah": dict(norm_trace=1, auto_start=1, res_first=50, res_last=30, adj_peak_norm=1, vp_ratio=1.732),
 }
 

@@ -233,3 +233,75 @@ def test_field_inversion_with_given_observed_picks():
     assert np.all(np.isfinite(out["vs"]))
     assert out["vs"].min() >= F.OUTCROP_VS_BOUNDS[0] and out["vs"].max() <= F.OUTCROP_VS_BOUNDS[1]
     assert out["records"][-1]["res0"] <= out["residual"][0]
+
+
+def test_engine_fk_filter_spectra_vs_oracle():
+    """The QadimahFault FK == 1 branch (fk_filter on each window, flipped
+    for the left side) with norm_trace: the engine's normalised spectra on its
+    own gathers against the oracle's RTr/RTl of the same gathers."""
+    prob, mod = P.plain_single_shot(nt=1500)
+    e = Engine(prob)
+    try:
+        e.set_variant({"fk_filter": 1, "norm_trace": 1})
+        e.set_wavelet(mod["s"])
+        e.set_model(mod["vp"], mod["vs"])
+        e.set_observed_picks(np.ones((e.npair, 1)), np.ones((e.npair, 1)))
+        e.run("forward")
+        e.run("dispersion")
+        spec = e.read("spectra")
+        gath = e.read("gathers")[0].astype(np.float64)
+        np_ = e.info()["np"]
+    finally:
+        e.close()
+    q = prob
+    xn = O.norm_trace(gath)
+    for col, side in ((0, "r"), (1, "l")):
+        ml, L, _ = O.rt_side(xn, side, 5, q["df"], q["dt"], np_, q["vmin"], q["vmax"], q["fmin"], q["fmax"], q["nt"],
+                             q["ng"], q["ds"], q["offset"], q["m"], q["M"], fk=True)
+        plain, _, _ = O.rt_side(xn, side, 5, q["df"], q["dt"], np_, q["vmin"], q["vmax"], q["fmin"], q["fmax"],
+                                q["nt"], q["ng"], q["ds"], q["offset"], q["m"], q["M"])
+        err = rel(spec[col].T, ml)
+        print(f"side {side} ({L} traces): fk spectra rel-L2 vs FP64 {err:.2e}; the filter moves them by "
+              f"{rel(plain, ml):.2e}")
+        assert rel(plain, ml) > 1e-3      # the filter does change the spectrum
+        assert err < 1e-4
+
+
+def test_dispersion_batch_fk_filter_vs_oracle(gained_chirp):
+    """fk_filter + norm_trace on the 600-trace chirp windows: picks within one
+    bin of the oracle's (long windows) and adjoint sources within 1e-4."""
+    prob, g, gg = gained_chirp
+    q = prob
+    var = {"fk_filter": 1, "norm_trace": 1}
+    D = DispersionBatch(prob)
+    try:
+        D.set_variant(var)
+        D.set_gathers(gg)
+        ones = np.ones((D.npair, NSHOT))
+        D.set_observed_picks(ones, ones)
+        picks = D.pick()
+        r = np.random.default_rng(13)
+        gr = np.round(r.normal(0, 5, (D.npair, NSHOT)))
+        gl = np.round(r.normal(0, 5, (D.npair, NSHOT)))
+        D.set_observed_picks(picks["cr_r"] - gr, picks["cr_l"] - gl)
+        D.pick()
+        adj, _ = D.adjoint(10.0)
+    finally:
+        D.close()
+    for s in range(NSHOT):
+        xn = O.norm_trace(gg[s].astype(np.float64))
+        sides = {}
+        for side, key in (("r", "cr_r"), ("l", "cr_l")):
+            ml, L, sfb = O.rt_side(xn, side, *_radon_args(q, s, D.np), fk=True)
+            sides[side] = sfb
+            if L < 200:
+                continue
+            _, cr = O.LHDispPick(ml, q["vmin"], np.ones(D.npair), 1, q["ini"])
+            d = np.max(np.abs(picks[key][:, s] - cr))
+            print(f"fk shot {s + 1} side {side}: max pick difference {d}")
+            assert d <= 1.0
+        a_ref = O.ADWDgrad_1(q["nt"], q["ng"], NSHOT, D.npair, s + 1, q["w"], q["m"], q["M"], 10.0, gr[:, s],
+                             gl[:, s], sides["r"], sides["l"])
+        e = rel(adj[s], a_ref)
+        print(f"fk shot {s + 1}: adjoint source rel-L2 vs FP64 {e:.3e}")
+        assert e < 1e-4

@@ -85,7 +85,8 @@ def test_variant_structs():
     q = F.variant_struct("qadimah")
     assert (q.res_first, q.res_last) == (50, 30)
     z = F.variant_struct("swd")
-    assert (z.norm_trace, z.auto_start, z.res_first, z.res_last, z.adj_peak_norm, z.vp_ratio) == (0, 0, 0, 0, 0, 0)
+    assert (z.norm_trace, z.auto_start, z.res_first, z.res_last, z.adj_peak_norm, z.vp_ratio, z.fk_filter) == (0,) * 7
+    assert F.variant_struct({"fk_filter": 1}).fk_filter == 1
     with pytest.raises(ValueError):
         F.variant_struct({"norm": 1})
 
@@ -127,3 +128,67 @@ def test_outcrop_manual_cube_fits_the_outcrop_problem():
         cr = F.manual_picks(cube, prob["fmin"], prob["fmax"], prob["df"], prob["offset"], 5)
         assert cr.shape == (problem_npair(prob), 60)
         assert np.all(np.isfinite(cr)) and cr.min() >= 1 and cr.max() <= 1000
+
+
+def _fk_brute(d):
+    """fk_filter.m's steps with explicit sums and explicit shift indices (1-based as in MATLAB)."""
+    ns, ntr = d.shape
+    nf, nk = 2 * ns, 2 * ntr
+    P = np.zeros((nf, nk), dtype=complex)
+    P[:ns, :ntr] = d
+    F2 = np.zeros((nf, nk), dtype=complex)
+    for u in range(nf):
+        for v in range(nk):
+            F2[u, v] = sum(P[t, x] * np.exp(-2j * np.pi * (u * t / nf + v * x / nk)) for t in range(ns) for x in range(ntr))
+    # fftshift of even sizes: shifted(i, j) = F2((i + nf/2) mod nf, (j + nk/2) mod nk)
+    S = np.array([[F2[(i + nf // 2) % nf, (j + nk // 2) % nk] for j in range(nk)] for i in range(nf)])
+    for i in range(1, ns + 1):                 # for i = 1:nfhalf, for j = 1:nkhalf
+        for j in range(1, ntr + 1):
+            S[i - 1, j - 1] = 0
+            S[i + ns - 1, j + ntr - 1] = 0
+    B = np.array([[S[(i + nf // 2) % nf, (j + nk // 2) % nk] for j in range(nk)] for i in range(nf)])
+    out = np.zeros((ns, ntr))
+    for t in range(ns):
+        for x in range(ntr):
+            out[t, x] = np.real(sum(B[u, v] * np.exp(2j * np.pi * (u * t / nf + v * x / nk))
+                                    for u in range(nf) for v in range(nk))) / (nf * nk)
+    return out
+
+
+@pytest.mark.parametrize("shape", [(5, 3), (6, 4), (4, 1)])
+def test_fk_filter_brute_force(shape):
+    d = np.random.default_rng(shape[0] * 7 + shape[1]).standard_normal(shape)
+    assert np.allclose(O.fk_filter(d), _fk_brute(d), rtol=0, atol=1e-12)
+    d2 = np.random.default_rng(99).standard_normal(shape)
+    assert np.allclose(O.fk_filter(2 * d - 3 * d2), 2 * O.fk_filter(d) - 3 * O.fk_filter(d2), atol=1e-12)
+
+
+@pytest.mark.parametrize("nt,L,ccn,c0,npair", [(40, 7, 64, 3, 9), (41, 8, 64, 0, 12), (50, 6, 30, 2, 7), (33, 1, 40, 1, 5)])
+def test_fk_spectrum_factorisation(nt, L, ccn, c0, npair):
+    """The form the device evaluates: with W the DFT at the bins (t < min(nt, ccn)),
+    N = 2nt, M = 2L, E / O the even / odd-t sums of W, Xe / Xo the even / odd-t
+    sums of each trace and H = U' X with U'[k, s] = -(1/N) sum_{t-s odd} W[t, k] cot(pi (t-s)/N):
+      W^T fk_filter(X) [:, l] = D[:, l]/2 - (1/M) sum_{l-l' odd} [(2/N)(O Xe[l'] + E Xo[l'])
+                                                                  + 2 cot(pi (l-l')/M) H[:, l']]."""
+    r = np.random.default_rng(nt + L)
+    X = r.standard_normal((nt, L))
+    nte = min(nt, ccn)
+    W = np.exp(-2j * np.pi * np.outer(np.arange(nte), c0 + np.arange(npair)) / ccn)
+    ref = W.T @ O.fk_filter(X)[:nte]
+    N, M = 2 * nt, 2 * L
+    D = W.T @ X[:nte]
+    t = np.arange(nte)
+    E, Od = W[t % 2 == 0].sum(0), W[t % 2 == 1].sum(0)
+    Xe, Xo = X[0::2].sum(0), X[1::2].sum(0)
+    tau = t[:, None] - np.arange(nt)[None, :]
+    K = np.zeros(tau.shape)
+    odd = np.abs(tau) % 2 == 1
+    K[odd] = -1.0 / np.tan(np.pi * tau[odd] / N) / N
+    H = (W.T @ K) @ X
+    Df = np.zeros((npair, L), dtype=complex)
+    for l in range(L):
+        acc = np.zeros(npair, dtype=complex)
+        for lp in range(1 - l % 2, L, 2):
+            acc += (2 / N) * (Od * Xe[lp] + E * Xo[lp]) + 2 / np.tan(np.pi * (l - lp) / M) * H[:, lp]
+        Df[:, l] = D[:, l] / 2 - acc / M
+    assert np.max(np.abs(Df - ref)) <= 1e-13 * np.max(np.abs(ref))
