@@ -33,14 +33,16 @@ def fd_bytes_per_launch(prob, nshots, adjoint):
     60 B per padded cell-step (5 state fields read + written, 5 coefficients
     read) plus, averaged over the in_wf-step snapshot cycle, 16 B per interior
     cell per snapshot step (forward: strain written; adjoint: strain read) and,
-    for the adjoint, the image read-modify-write (3 images x 2 x 8 B, FP64)."""
+    for the adjoint, the image update: an FP32 read-modify-write of the 3
+    partial images (24 B) plus, every 16th imaging step, the FP64
+    read-modify-write of the 3 images (48 B / 16)."""
     nzbc = prob["nz"] + 2 + prob["nbc"]
     nxbc = prob["nx"] + 2 * prob["nbc"]
     cells = nzbc * nxbc
     inner = prob["nz"] * prob["nx"]
     per = 60.0 * cells + 16.0 * inner / prob["in_wf"]
     if adjoint:
-        per += 48.0 * inner / prob["in_wf"]
+        per += (24.0 + 48.0 / 16) * inner / prob["in_wf"]
     return per * nshots
 
 

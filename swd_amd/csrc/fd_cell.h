@@ -53,4 +53,21 @@ __device__ __forceinline__ T cell_adj_shear(T temp, T s, T dtx, T d1, T d2)
     return fma(-dtx, d2, fma(-dtx, d1, temp * s));
 }
 
+// Imaging condition (core/eigen_e2drtm_single.cpp:274-280) at one cell in
+// FP64: c0 -> cl_img, c1 -> cm_img, c2 -> illum.
+__device__ __forceinline__ void image_terms(double Fux, double Fwz, double Fsh, double X, double Z, double XZ,
+                                            double& c0, double& c1, double& c2)
+{
+    c0 = (Fux + Fwz) * (X + Z);
+    c1 = 2.0 * (X * Fux + Z * Fwz + 0.5 * XZ * Fsh);
+    c2 = (Fux + Fwz) * (Fux + Fwz);
+}
+
+// An imaging step into the FP32 partial sum p, and the flush of p plus the
+// step into the FP64 total (every kImgFlush imaging steps); every kernel
+// that accumulates images uses these, so the sums do not depend on the
+// launch shape.
+__device__ __forceinline__ float image_partial(float p, double c) { return (float)((double)p + c); }
+__device__ __forceinline__ double image_flush(double tot, float p, double c) { return tot + ((double)p + c); }
+
 }  // namespace swd

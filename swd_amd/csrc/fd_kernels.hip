@@ -26,6 +26,8 @@
 //   xx = temp*xx + (ca*fux + cl*bwz)*dtx; zz = temp*zz + (ca*bwz + cl*fux)*dtx;
 //   xz = temp*xz + cm1*(fuz+bwx)*dtx;  zz[pt,:] = 0                    (:334-340)
 // Adjoint step, core/eigen_e2drtm_single.cpp:181-283 (it = nt-2 .. 0).
+#include <type_traits>
+
 #include "fd_cell.h"
 #include "swd_internal.h"
 
@@ -310,11 +312,27 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_march(
                 } else {
                     Fux = sn[0]; Fwz = sn[3 * plane]; Fsh = (double)(sn[plane] + sn[2 * plane]);
                 }
-                const double X = xxn, Z = zzn, XZ = xzn;
+                double c0, c1, c2;
+                image_terms(Fux, Fwz, Fsh, xxn, zzn, xzn, c0, c1, c2);
                 double* im = J.img + io;
-                im[0] += (Fux + Fwz) * (X + Z);
-                im[plane] += 2.0 * (X * Fux + Z * Fwz + 0.5 * XZ * Fsh);
-                im[2 * plane] += (Fux + Fwz) * (Fux + Fwz);
+                if constexpr (std::is_same<T, float>::value) {
+                    // FP32 partials, flushed into the FP64 images every kImgFlush imaging steps
+                    float* pp = J.imgp + io;
+                    if (((it / J.in_wf) % kImgFlush) == 0) {
+                        im[0] = image_flush(im[0], pp[0], c0);
+                        im[plane] = image_flush(im[plane], pp[plane], c1);
+                        im[2 * plane] = image_flush(im[2 * plane], pp[2 * plane], c2);
+                        pp[0] = pp[plane] = pp[2 * plane] = 0.f;
+                    } else {
+                        pp[0] = image_partial(pp[0], c0);
+                        pp[plane] = image_partial(pp[plane], c1);
+                        pp[2 * plane] = image_partial(pp[2 * plane], c2);
+                    }
+                } else {
+                    im[0] += c0;
+                    im[plane] += c1;
+                    im[2 * plane] += c2;
+                }
             }
         }
         xx_c = xx_n;

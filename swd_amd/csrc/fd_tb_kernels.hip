@@ -18,6 +18,8 @@
 // as the reference's never-updated ring, without per-lane predicates.
 // Only source type 5 (the hard vertical-velocity source SWD uses) is
 // handled here; other source types go through the single-step kernels.
+#include <type_traits>
+
 #include "fd_cell.h"
 #include "swd_internal.h"
 
@@ -86,11 +88,12 @@ struct RowLoad {
 };
 
 // Snapshot and image values of one imaging row for this lane, loaded one
-// row ahead like the state (raw snapshot fields, current FP64 images).
+// row ahead like the state: raw snapshot fields and the current image sums
+// (float kernels: the FP32 partials; double kernels: the FP64 images).
 template <typename T>
 struct ImgRow {
     T s0, s1, s2, s3;
-    double m0, m1, m2;
+    T m0, m1, m2;
 };
 
 template <typename T, int NT>
@@ -302,6 +305,11 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
     const int in_wf = J.in_wf, nsf = J.nsf, nzi = J.nz, nxi = J.nx, nbc = J.nbc;
     const gptr<const T> snap = (gptr<const T>)J.snap;
     const gptr<double> img = (gptr<double>)J.img;
+    // float kernels sum the images in FP32 partials (J.imgp), flushed into
+    // the FP64 images every kImgFlush imaging steps; double kernels add into
+    // the FP64 images directly
+    constexpr bool kPart = std::is_same<T, float>::value;
+    const gptr<T> imgp = (gptr<T>)(kPart ? (T*)J.imgp : (T*)J.img);
 
     int gi = -1;
     {
@@ -327,6 +335,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
     for (int l = 0; l < NT; ++l)
         if (valid[l] && (itl[l] % in_wf) == 0) { limg = l; ++nimg; }
     const bool pf_img = nimg == 1;
+    const bool flush = pf_img && ((itl[limg] / in_wf) % kImgFlush) == 0;
     const size_t iplane = (size_t)nzi * nxi;
     const gptr<const T> snb = snap + (pf_img ? (size_t)(itl[limg] / in_wf) * nsf * iplane : 0);
     auto img_load = [&](ImgRow<T>& q, const int i) {
@@ -338,7 +347,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
             q.s1 = sn[iplane];
             q.s2 = sn[2 * iplane];
             if (nsf == 4) q.s3 = sn[3 * iplane];
-            const gptr<const double> im = img + io;
+            const gptr<const T> im = imgp + io;
             q.m0 = im[0];
             q.m1 = im[iplane];
             q.m2 = im[2 * iplane];
@@ -394,11 +403,27 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
                     } else {
                         Fux = icur.s0; Fwz = icur.s3; Fsh = (double)(icur.s1 + icur.s2);
                     }
-                    const double X = oxx, Z = ozz, XZ = oxz;
-                    const gptr<double> im = img + (size_t)(r - pt - 1) * nxi + (x - nbc);
-                    im[0] = icur.m0 + (Fux + Fwz) * (X + Z);
-                    im[iplane] = icur.m1 + 2.0 * (X * Fux + Z * Fwz + 0.5 * XZ * Fsh);
-                    im[2 * iplane] = icur.m2 + (Fux + Fwz) * (Fux + Fwz);
+                    double c0, c1, c2;
+                    image_terms(Fux, Fwz, Fsh, oxx, ozz, oxz, c0, c1, c2);
+                    const size_t io = (size_t)(r - pt - 1) * nxi + (x - nbc);
+                    const gptr<double> im = img + io;
+                    if constexpr (kPart) {
+                        const gptr<T> pp = imgp + io;
+                        if (flush) {
+                            im[0] = image_flush(im[0], icur.m0, c0);
+                            im[iplane] = image_flush(im[iplane], icur.m1, c1);
+                            im[2 * iplane] = image_flush(im[2 * iplane], icur.m2, c2);
+                            pp[0] = pp[iplane] = pp[2 * iplane] = T(0);
+                        } else {
+                            pp[0] = image_partial(icur.m0, c0);
+                            pp[iplane] = image_partial(icur.m1, c1);
+                            pp[2 * iplane] = image_partial(icur.m2, c2);
+                        }
+                    } else {
+                        im[0] = (double)icur.m0 + c0;
+                        im[iplane] = (double)icur.m1 + c1;
+                        im[2 * iplane] = (double)icur.m2 + c2;
+                    }
                 }
             } else if (out && valid[l] && r >= z0 && r < z1 && (itl[l] % in_wf) == 0 && xint && r > pt &&
                        r <= pt + nzi) {
@@ -411,11 +436,26 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
                 } else {
                     Fux = sn[0]; Fwz = sn[3 * plane]; Fsh = (double)(sn[plane] + sn[2 * plane]);
                 }
-                const double X = oxx, Z = ozz, XZ = oxz;
+                double c0, c1, c2;
+                image_terms(Fux, Fwz, Fsh, oxx, ozz, oxz, c0, c1, c2);
                 const gptr<double> im = img + io;
-                im[0] += (Fux + Fwz) * (X + Z);
-                im[plane] += 2.0 * (X * Fux + Z * Fwz + 0.5 * XZ * Fsh);
-                im[2 * plane] += (Fux + Fwz) * (Fux + Fwz);
+                if constexpr (kPart) {
+                    const gptr<T> pp = imgp + io;
+                    if (((itl[l] / in_wf) % kImgFlush) == 0) {
+                        im[0] = image_flush(im[0], pp[0], c0);
+                        im[plane] = image_flush(im[plane], pp[plane], c1);
+                        im[2 * plane] = image_flush(im[2 * plane], pp[2 * plane], c2);
+                        pp[0] = pp[plane] = pp[2 * plane] = T(0);
+                    } else {
+                        pp[0] = image_partial(pp[0], c0);
+                        pp[plane] = image_partial(pp[plane], c1);
+                        pp[2 * plane] = image_partial(pp[2 * plane], c2);
+                    }
+                } else {
+                    im[0] += c0;
+                    im[plane] += c1;
+                    im[2 * plane] += c2;
+                }
             }
             cxx[l] = ixx; czz[l] = izz; cxz[l] = ixz; cwin[l] = iw;
             cclxx[l] = clxx; ccazz[l] = cazz; ccmxz[l] = cmxz;

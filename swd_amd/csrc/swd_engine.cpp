@@ -110,6 +110,7 @@ int Engine::create(const swd_problem* prob, hipStream_t stream)
         (rc = traces.alloc(S * ntr * sizeof(float))) ||
         (rc = snaps.alloc(S * (size_t)nrec * 3 * splane * sizeof(float))) ||
         (rc = adjsrc.alloc(S * ntr * sizeof(float))) || (rc = imgs.alloc(S * 3 * splane * sizeof(double))) ||
+        (rc = imgp.alloc(S * 3 * splane * sizeof(float))) ||
         (rc = outs.alloc(4 * splane * sizeof(double))) || (rc = vp_d.alloc(splane * sizeof(double))) ||
         (rc = geo.alloc((size_t)nzbc * nxbc * sizeof(double))) || (rc = vs_cur.alloc(splane * sizeof(double))) ||
         (rc = vs_tr.alloc(splane * sizeof(double))) || (rc = dk_sm.alloc(splane * sizeof(double))) ||
@@ -127,7 +128,7 @@ int Engine::create(const swd_problem* prob, hipStream_t stream)
 
 size_t Engine::footprint_bytes() const
 {
-    const DevBuf* b[] = {&coefs, &vs_d, &wavelet, &state, &strain, &traces, &snaps, &adjsrc, &imgs, &outs, &djobs, &dbmap,
+    const DevBuf* b[] = {&coefs, &vs_d, &wavelet, &state, &strain, &traces, &snaps, &adjsrc, &imgs, &imgp, &outs, &djobs, &dbmap,
                          &disp.W, &disp.D, &disp.X, &disp.MM, &disp.ml, &disp.mlr, &disp.GMM, &disp.GD,
                          &disp.Bm, &disp.crpre, &disp.cr0, &disp.res, &disp.status, &vp_d, &geo, &vs_cur,
                          &vs_tr, &dk_sm, &stats, &disp.dcols};
@@ -327,6 +328,7 @@ void Engine::make_jobs(std::vector<FdJob<float>>& jobs, bool adj, bool snapshots
         J.trace_u = nullptr;
         J.snap = snaps.as<float>() + (size_t)s * nrec * 3 * splane;
         J.img = adj ? imgs.as<double>() + (size_t)s * 3 * splane : nullptr;
+        J.imgp = adj ? imgp.as<float>() + (size_t)s * 3 * splane : nullptr;
         J.nzbc = nzbc; J.nxbc = nxbc; J.pitch = pitch; J.nz = p.nz; J.nx = p.nx; J.nbc = p.nbc; J.pt = 1;
         J.sz = sz[s] + pt_m - 1; J.sx = sx[s] + p.nbc - 1; J.stype = adj ? 0 : 5;
         J.gz = p.gz + pt_m - 1; J.gx = p.gx0 + p.nbc - 1; J.dg = p.dg; J.ng = p.ng;
@@ -345,7 +347,10 @@ int run_fd_batch(const std::vector<Engine*>& engines, bool adj, bool snapshots, 
         e->make_jobs(jobs, adj, snapshots);
         steps = std::max(steps, adj ? e->p.nt - 1 : e->p.nt);
         SWD_HIP(hipMemsetAsync(e->state.p, 0, e->state.bytes, st));
-        if (adj) SWD_HIP(hipMemsetAsync(e->imgs.p, 0, e->imgs.bytes, st));
+        if (adj) {
+            SWD_HIP(hipMemsetAsync(e->imgs.p, 0, e->imgs.bytes, st));
+            SWD_HIP(hipMemsetAsync(e->imgp.p, 0, e->imgp.bytes, st));
+        }
         else SWD_HIP(hipMemsetAsync(e->traces.p, 0, e->traces.bytes, st));
     }
     const Engine* e0 = engines[0];
@@ -377,7 +382,10 @@ int Engine::run_fd(bool adj, bool snapshots)
     SWD_HIP(hipMemsetAsync(strain.p, 0, strain.bytes, st));
     std::vector<FdJob<float>> jobs;
     make_jobs(jobs, adj, snapshots);
-    if (adj) SWD_HIP(hipMemsetAsync(imgs.p, 0, imgs.bytes, st));
+    if (adj) {
+        SWD_HIP(hipMemsetAsync(imgs.p, 0, imgs.bytes, st));
+        SWD_HIP(hipMemsetAsync(imgp.p, 0, imgp.bytes, st));
+    }
     else SWD_HIP(hipMemsetAsync(traces.p, 0, traces.bytes, st));
     float* w[4] = {strain.as<float>(), strain.as<float>() + plane, strain.as<float>() + 2 * plane,
                    strain.as<float>() + 3 * plane};

@@ -54,6 +54,7 @@ public:
     DevBuf snaps;     // [shot][nrec][3][nz][nx] float
     DevBuf adjsrc;    // [shot][t][g] float
     DevBuf imgs;      // [shot][3][nz][nx] double
+    DevBuf imgp;      // [shot][3][nz][nx] float, partial image sums of the fused kernels
     DevBuf outs;      // dk_vs, g_cl, g_cm, g_il (nz x nx double each)
     DevBuf djobs, dbmap;
     DevBuf vp_d;      // nz x nx double, the current vp (column-major)

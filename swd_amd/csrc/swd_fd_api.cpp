@@ -307,9 +307,13 @@ static int e2drtm_impl(const T* iv, const T* const coef_in[7], const T* adj, con
         if ((rc = snaps.upload(h.data(), h.size() * sizeof(T), st.s))) return rc;
         SWD_HIP(hipStreamSynchronize(st.s));
     }
-    if ((rc = state.alloc(10 * plane * sizeof(T))) || (rc = img.alloc(3 * splane * sizeof(double)))) return rc;
+    DevBuf imgp;
+    if ((rc = state.alloc(10 * plane * sizeof(T))) || (rc = img.alloc(3 * splane * sizeof(double))) ||
+        (rc = imgp.alloc(3 * splane * sizeof(float))))
+        return rc;
     SWD_HIP(hipMemsetAsync(state.p, 0, 10 * plane * sizeof(T), st.s));
     SWD_HIP(hipMemsetAsync(img.p, 0, 3 * splane * sizeof(double), st.s));
+    SWD_HIP(hipMemsetAsync(imgp.p, 0, 3 * splane * sizeof(float), st.s));
     T* C = (T*)coefs.p;
     T* S = (T*)state.p;
     FdJob<T> J;
@@ -325,6 +329,7 @@ static int e2drtm_impl(const T* iv, const T* const coef_in[7], const T* adj, con
     J.src = (const T*)dadj.p;
     J.snap = (T*)snaps.p;
     J.img = (double*)img.p;
+    J.imgp = (float*)imgp.p;
     J.nzbc = d.nzbc; J.nxbc = d.nxbc; J.pitch = P; J.nz = d.nz; J.nx = d.nx; J.nbc = d.nbc; J.pt = d.pt;
     J.gz = d.gz; J.gx = d.gx; J.dg = d.dg; J.ng = d.ng;
     J.nt = d.nt; J.in_wf = d.in_wf; J.start = 0; J.nsf = 4;

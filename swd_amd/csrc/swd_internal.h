@@ -63,6 +63,11 @@ struct FdJob {
     T* snap;
     // adjoint images [3][nz][nx] (cl_img, cm_img, illum), accumulated in double
     double* img;
+    // FP32 partial sums of the images, [3][nz][nx] (required by the fused
+    // float kernels, unused by the others): every imaging step adds into
+    // them, and imaging step k with k % kImgFlush == 0 (the last one, it = 0,
+    // included) moves them into img and clears them
+    float* imgp;
     int nzbc, nxbc, pitch, nz, nx, nbc, pt;
     int sz, sx, stype;      // 0-based source cell and source type (1..5; 0 = none)
     int gz, gx, dg, ng;     // 0-based receiver row / first column, spacing, count
@@ -73,6 +78,9 @@ struct FdJob {
     int tiles_x, tiles_z, tz;  // march tiling (fused kernels)
     T dtx, dt;
 };
+
+// Imaging steps per FP64 flush of the FP32 partial images (FdJob::imgp).
+constexpr int kImgFlush = 16;
 
 // Block -> (job, first tile) map entry for fused launches.
 struct BlockMap {

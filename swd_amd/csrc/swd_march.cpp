@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <type_traits>
 #include <vector>
 
 #include "swd_host.h"
@@ -76,6 +77,7 @@ int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int steps, 
 {
     bool tb_ok = nt_block >= 2;
     for (const auto& J : jobs) {
+        if (adj && std::is_same<T, float>::value && !J.imgp) { set_error("float adjoint jobs need partial images"); return kErrArg; }
         if (J.stype != 0 && J.stype != 5) tb_ok = false;
         if (J.start != 0 || J.launch0 != 0) { set_error("run_march jobs must start together"); return kErrArg; }
     }

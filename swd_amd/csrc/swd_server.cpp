@@ -581,7 +581,10 @@ int Server::rebuild(int ph)
             Engine& g = *r->eng;
             SWD_HIP(hipMemsetAsync(g.state.p, 0, g.state.bytes, fd[ph]));
             if (ph == 0) SWD_HIP(hipMemsetAsync(g.traces.p, 0, g.traces.bytes, fd[ph]));
-            else SWD_HIP(hipMemsetAsync(g.imgs.p, 0, g.imgs.bytes, fd[ph]));
+            else {
+                SWD_HIP(hipMemsetAsync(g.imgs.p, 0, g.imgs.bytes, fd[ph]));
+                SWD_HIP(hipMemsetAsync(g.imgp.p, 0, g.imgp.bytes, fd[ph]));
+            }
         }
         for (auto& J : r->jobs) {
             J.start = gstep[ph] - r->done * NT;
