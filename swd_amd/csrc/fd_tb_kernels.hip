@@ -92,7 +92,7 @@ struct RowLoad {
 // (float kernels: the FP32 partials; double kernels: the FP64 images).
 template <typename T>
 struct ImgRow {
-    T s0, s1, s2, s3;
+    T fux, fwz, fsh;  // snapshot fields of the imaging condition (fsh = fuz + bwx)
     T m0, m1, m2;
 };
 
@@ -343,10 +343,14 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
         if (pf_img && r >= z0 && r < z1 && r > pt && r <= pt + nzi && img_lane) {
             const size_t io = (size_t)(r - pt - 1) * nxi + (x - nbc);
             const gptr<const T> sn = snb + io;
-            q.s0 = sn[0];
-            q.s1 = sn[iplane];
-            q.s2 = sn[2 * iplane];
-            if (nsf == 4) q.s3 = sn[3 * iplane];
+            q.fux = sn[0];
+            if (nsf == 3) {   // [fux, bwz, fuz + bwx]
+                q.fwz = sn[iplane];
+                q.fsh = sn[2 * iplane];
+            } else {          // [fux, fuz, bwx, bwz]
+                q.fwz = sn[3 * iplane];
+                q.fsh = sn[iplane] + sn[2 * iplane];
+            }
             const gptr<const T> im = imgp + io;
             q.m0 = im[0];
             q.m1 = im[iplane];
@@ -354,12 +358,13 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
         }
     };
 
-    // carried per level: previous input row (stress, its coefficient products,
-    // old ww), previous outputs u_l(r), w_l(r-1)
-    T cxx[NT], czz[NT], cxz[NT], cwin[NT], cclxx[NT], ccazz[NT], ccmxz[NT], cuo[NT], cwo[NT];
+    // carried per level: previous input row (stress, old ww), previous
+    // outputs u_l(r), w_l(r-1).  The previous row's coefficient products are
+    // formed again from it and its coefficients (ring[l + 1]): the same
+    // products, without carrying 3 NT more registers.
+    T cxx[NT], czz[NT], cxz[NT], cwin[NT], cuo[NT], cwo[NT];
 #pragma unroll
-    for (int l = 0; l < NT; ++l)
-        cxx[l] = czz[l] = cxz[l] = cwin[l] = cclxx[l] = ccazz[l] = ccmxz[l] = cuo[l] = cwo[l] = T(0);
+    for (int l = 0; l < NT; ++l) cxx[l] = czz[l] = cxz[l] = cwin[l] = cuo[l] = cwo[l] = T(0);
     CoefRow<T> ring[NT + 1];
 #pragma unroll
     for (int k = 0; k <= NT; ++k) ring[k] = CoefRow<T>{T(0), T(0), T(0), T(0), T(0), T(0)};
@@ -381,10 +386,12 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
                     cmxz = cq.cm1 * ixz;
             // uu_b(q)
             const T caxx_l = tb_left(caxx), clzz_l = tb_left(clzz);
-            const T u_q = cell_adj_vel(cq.temp, iu, cq.b, (caxx - caxx_l) + (clzz - clzz_l) + (cmxz - ccmxz[l]));
+            // coefficient products of the previous input row r
+            const T pclxx = cr_.cl * cxx[l], pcazz = cr_.ca * czz[l], pcmxz = cr_.cm1 * cxz[l];
+            const T u_q = cell_adj_vel(cq.temp, iu, cq.b, (caxx - caxx_l) + (clzz - clzz_l) + (cmxz - pcmxz));
             // ww_b(r) (+ adjoint source at the receiver row)
-            const T cmxz_r = tb_right(ccmxz[l]);
-            T w_r = cell_adj_vel(cr_.temp, cwin[l], cr_.b1, (clxx - cclxx[l]) + (cazz - ccazz[l]) + (cmxz_r - ccmxz[l]));
+            const T cmxz_r = tb_right(pcmxz);
+            T w_r = cell_adj_vel(cr_.temp, cwin[l], cr_.b1, (clxx - pclxx) + (cazz - pcazz) + (cmxz_r - pcmxz));
             if (r == gz) w_r += inj[l];
             // adjoint stress(r)
             const T u_r = cuo[l];
@@ -397,14 +404,8 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
             if (pf_img) {
                 if (l == limg && r >= z0 && r < z1 && r > pt && r <= pt + nzi && img_lane) {
                     // prefetched snapshot and image values of this row
-                    double Fux, Fwz, Fsh;
-                    if (nsf == 3) {
-                        Fux = icur.s0; Fwz = icur.s1; Fsh = icur.s2;
-                    } else {
-                        Fux = icur.s0; Fwz = icur.s3; Fsh = (double)(icur.s1 + icur.s2);
-                    }
                     double c0, c1, c2;
-                    image_terms(Fux, Fwz, Fsh, oxx, ozz, oxz, c0, c1, c2);
+                    image_terms(icur.fux, icur.fwz, icur.fsh, oxx, ozz, oxz, c0, c1, c2);
                     const size_t io = (size_t)(r - pt - 1) * nxi + (x - nbc);
                     const gptr<double> im = img + io;
                     if constexpr (kPart) {
@@ -458,7 +459,6 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
                 }
             }
             cxx[l] = ixx; czz[l] = izz; cxz[l] = ixz; cwin[l] = iw;
-            cclxx[l] = clxx; ccazz[l] = cazz; ccmxz[l] = cmxz;
             cuo[l] = u_q;
             cwo[l] = w_r;
             ixx = oxx; izz = ozz; ixz = oxz; iu = u_r; iw = w_r;
