@@ -66,8 +66,23 @@ __device__ __forceinline__ void image_terms(double Fux, double Fwz, double Fsh, 
 // An imaging step into the FP32 partial sum p, and the flush of p plus the
 // step into the FP64 total (every kImgFlush imaging steps); every kernel
 // that accumulates images uses these, so the sums do not depend on the
-// launch shape.
-__device__ __forceinline__ float image_partial(float p, double c) { return (float)((double)p + c); }
+// launch shape.  A sum below FP32's normal range (0 < |v| < 2^-126, e.g.
+// the squared strain of a cell the wavefield barely reaches) would lose its
+// relative precision, or vanish, as a float: it goes to the FP64 total at
+// once and the partial restarts from 0.  Otherwise the ratios
+// cl_img ./ illum of weakly lit cells would differ from FP64 (0/0 or x/0
+// where FP64 gives a finite ratio).
+__device__ __forceinline__ bool image_tiny(double v) { return (v != 0.0) & (fabs(v) < 0x1p-126); }
+template <typename DP>
+__device__ __forceinline__ float image_partial(float p, double c, DP tot)
+{
+    const double v = (double)p + c;
+    if (image_tiny(v)) {
+        *tot += v;
+        return 0.f;
+    }
+    return (float)v;
+}
 __device__ __forceinline__ double image_flush(double tot, float p, double c) { return tot + ((double)p + c); }
 
 }  // namespace swd

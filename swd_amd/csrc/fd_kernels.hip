@@ -324,9 +324,9 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_march(
                         im[2 * plane] = image_flush(im[2 * plane], pp[2 * plane], c2);
                         pp[0] = pp[plane] = pp[2 * plane] = 0.f;
                     } else {
-                        pp[0] = image_partial(pp[0], c0);
-                        pp[plane] = image_partial(pp[plane], c1);
-                        pp[2 * plane] = image_partial(pp[2 * plane], c2);
+                        pp[0] = image_partial(pp[0], c0, im);
+                        pp[plane] = image_partial(pp[plane], c1, im + plane);
+                        pp[2 * plane] = image_partial(pp[2 * plane], c2, im + 2 * plane);
                     }
                 } else {
                     im[0] += c0;

@@ -416,9 +416,18 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
                             im[2 * iplane] = image_flush(im[2 * iplane], icur.m2, c2);
                             pp[0] = pp[iplane] = pp[2 * iplane] = T(0);
                         } else {
-                            pp[0] = image_partial(icur.m0, c0);
-                            pp[iplane] = image_partial(icur.m1, c1);
-                            pp[2 * iplane] = image_partial(icur.m2, c2);
+                            // image_partial for the three images, with one test on the common path
+                            double v0 = (double)icur.m0 + c0, v1 = (double)icur.m1 + c1, v2 = (double)icur.m2 + c2;
+                            // a spill is the only access to that FP64 image in this launch (one
+                            // imaging level per launch here), so it goes as an atomic add without
+                            // return: no wait for it on the path
+                            double* imd = (double*)im;
+                            if (image_tiny(v0)) { atomicAdd(imd, v0); v0 = 0.0; }
+                            if (image_tiny(v1)) { atomicAdd(imd + iplane, v1); v1 = 0.0; }
+                            if (image_tiny(v2)) { atomicAdd(imd + 2 * iplane, v2); v2 = 0.0; }
+                            pp[0] = (float)v0;
+                            pp[iplane] = (float)v1;
+                            pp[2 * iplane] = (float)v2;
                         }
                     } else {
                         im[0] = (double)icur.m0 + c0;
@@ -448,9 +457,9 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
                         im[2 * plane] = image_flush(im[2 * plane], pp[2 * plane], c2);
                         pp[0] = pp[plane] = pp[2 * plane] = T(0);
                     } else {
-                        pp[0] = image_partial(pp[0], c0);
-                        pp[plane] = image_partial(pp[plane], c1);
-                        pp[2 * plane] = image_partial(pp[2 * plane], c2);
+                        pp[0] = image_partial(pp[0], c0, im);
+                        pp[plane] = image_partial(pp[plane], c1, im + plane);
+                        pp[2 * plane] = image_partial(pp[2 * plane], c2, im + 2 * plane);
                     }
                 } else {
                     im[0] += c0;

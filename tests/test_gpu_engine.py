@@ -115,6 +115,38 @@ def test_engine_stages_given_oracle_picks(single):
     e.close()
 
 
+def test_weakly_lit_cells_match_fp64():
+    """A short record (nt 1000) leaves cells the wavefield barely reaches.
+    Where the strains are FP32 numbers (illum above 1e-60 in FP64) the image
+    sums keep even their squares (sub-FP32 sums spill into FP64), so dk_vs =
+    f(cl_img ./ illum, ...) is finite and within 1e-4 of the oracle there.
+    Cells whose FP64 wavefield is below FP32's range (as in the reference's
+    _single MEX) read zero strain: 0/0 there, and nowhere else."""
+    prob, mod = P.plain_single_shot(nt=1000)
+    cr0_r, cr0_l = oracle_observed(prob, mod, [5])
+    dk_ref, shots, tot = oracle_gradient(prob, mod, [5], cr0_r, cr0_l)
+    o = shots[0]
+    e = _engine(prob, mod, cr0_r, cr0_l)
+    e.run("forward")
+    e.run("dispersion")
+    e.set_picks(o["cr_r"][:, None], o["cr_l"][:, None])
+    e.run("adjoint_source")
+    e.run("adjoint")
+    e.run("assemble")
+    dk = e.read("gradient")[0]
+    e.close()
+    il = tot[2]
+    lit = il > 1e-60
+    nonfin = ~np.isfinite(dk)
+    sub = (il > 0) & (il < 1e-38)
+    err = float(np.linalg.norm(dk[lit] - dk_ref[lit]) / np.linalg.norm(dk_ref[lit]))
+    print(f"cells: {il.size}, FP64 illum below FP32 range {int(sub.sum())}, above 1e-60 {int(lit.sum())}, "
+          f"non-finite on the GPU {int(nonfin.sum())} (largest FP64 illum among them "
+          f"{(il[nonfin].max() if nonfin.any() else 0):.1e}); dk_vs rel-L2 over cells above 1e-60 {err:.2e}")
+    assert not nonfin[lit].any()
+    assert err < 1e-4
+
+
 def test_engine_three_shots_vs_oracle():
     nz, nx = 30, 120
     vs = np.tile(np.linspace(220, 420, nz)[:, None], (1, nx))
