@@ -41,7 +41,12 @@ __global__ void dft_table_kernel(float* W, int nt_eff, int npair, int c0, int cc
 // ---------------------------------------------------------------------------
 // Strided batched FP32 GEMM on MFMA 16x16x4:
 //   C(b, m, n) = alpha * sum_k A(b, m, k) B(b, k, n)
-// 64x64 tile per 256-thread block, 4 wavefronts x (32x32), BK = 16.
+// 64x64 tile per 256-thread block, 4 wavefronts x (32x32).  Two kernels
+// with the same MFMA chain (plain ascending k), hence bitwise equal results:
+//   gemm_f32_kernel: K tiles of 16 through one LDS buffer (short K);
+//   gemm_f32_db:     K tiles of 32 double-buffered in LDS, the next tile's
+//                    global loads in flight during the current tile's MFMAs,
+//                    one barrier per tile (long K: the DFT at the bins).
 // ---------------------------------------------------------------------------
 constexpr int GBM = 64, GBN = 64, GBK = 16;
 
@@ -87,6 +92,99 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(GemmArgs g)
             acc[1][1] = mfma16x16x4(a1, b1, acc[1][1]);
         }
         __syncthreads();
+    }
+    float* C = g.C + (size_t)b * g.sCb;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gm = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+                const int gn = n0 + wn + j * 16 + (lane & 15);
+                if (gm < g.M && gn < g.N) C[(size_t)gm * g.sCm + (size_t)gn * g.sCn] = g.alpha * acc[i][j][r];
+            }
+}
+
+// LDS layout of a double-buffered operand tile, chosen at compile time by the
+// operand's global layout so that the coalesced stores and the fragment reads
+// (16 rows m at k and at k+1) are both free of bank conflicts:
+//   m (or n) contiguous in memory: [k][m], rows 80 floats apart (80 = 16 mod 32);
+//   k contiguous in memory:        [m][k], rows 34 floats apart (34 = 2 mod 32).
+constexpr int GDK = 32, GDS = 80, GDT = 34, GDBUF = GDK * GDS;  // GDBUF >= 64 * GDT
+template <bool MF>
+__device__ __forceinline__ int lds_off(int mm, int kk) { return MF ? kk * GDS + mm : mm * GDT + kk; }
+
+template <bool AMF, bool BNF>
+__global__ __launch_bounds__(256) void gemm_f32_db(GemmArgs g)
+{
+    __shared__ float As[2][GDBUF];
+    __shared__ float Bs[2][GDBUF];
+    const int b = blockIdx.z;
+    const int m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const float* A = g.A + (size_t)b * g.sAb;
+    const float* B = g.B + (size_t)b * g.sBb;
+    const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+    v4f acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    // element q of this thread's share of a 64 x 32 tile: (row of m or n, k);
+    // consecutive threads walk the operand's contiguous index
+    auto a_at = [&](int q, int& mm, int& kk) {
+        const int e = tid + 256 * q;
+        if (AMF) { mm = e % GBM; kk = e / GBM; } else { kk = e % GDK; mm = e / GDK; }
+    };
+    auto b_at = [&](int q, int& nn, int& kk) {
+        const int e = tid + 256 * q;
+        if (BNF) { nn = e % GBN; kk = e / GBN; } else { kk = e % GDK; nn = e / GDK; }
+    };
+    float ra[8], rb[8];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            int mm, kk, nn, kb;
+            a_at(q, mm, kk);
+            const int gm = m0 + mm, gk = k0 + kk;
+            ra[q] = (gm < g.M && gk < g.K) ? A[(size_t)gm * g.sAm + (size_t)gk * g.sAk] : 0.f;
+            b_at(q, nn, kb);
+            const int gn = n0 + nn, gk2 = k0 + kb;
+            rb[q] = (gn < g.N && gk2 < g.K) ? B[(size_t)gk2 * g.sBk + (size_t)gn * g.sBn] : 0.f;
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            int mm, kk, nn, kb;
+            a_at(q, mm, kk);
+            As[buf][lds_off<AMF>(mm, kk)] = ra[q];
+            b_at(q, nn, kb);
+            Bs[buf][lds_off<BNF>(nn, kb)] = rb[q];
+        }
+    };
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    int cur = 0;
+    const int l16 = lane & 15, lk = lane >> 4;
+    for (int k0 = 0; k0 < g.K; k0 += GDK) {
+        const bool more = k0 + GDK < g.K;
+        if (more) gload(k0 + GDK);
+#pragma unroll
+        for (int kk = 0; kk < GDK; kk += 4) {
+            const int kr = kk + lk;
+            const float a0 = As[cur][lds_off<AMF>(wm + l16, kr)];
+            const float a1 = As[cur][lds_off<AMF>(wm + 16 + l16, kr)];
+            const float b0 = Bs[cur][lds_off<BNF>(wn + l16, kr)];
+            const float b1 = Bs[cur][lds_off<BNF>(wn + 16 + l16, kr)];
+            acc[0][0] = mfma16x16x4(a0, b0, acc[0][0]);
+            acc[0][1] = mfma16x16x4(a0, b1, acc[0][1]);
+            acc[1][0] = mfma16x16x4(a1, b0, acc[1][0]);
+            acc[1][1] = mfma16x16x4(a1, b1, acc[1][1]);
+        }
+        if (more) lstore(cur ^ 1);
+        __syncthreads();
+        cur ^= 1;
     }
     float* C = g.C + (size_t)b * g.sCb;
 #pragma unroll
@@ -875,7 +973,11 @@ int launch_gemm_f32(hipStream_t st, const GemmArgs& g, int batch)
     if (g.M <= 0 || g.N <= 0 || batch <= 0) return kOk;
     const dim3 grid(div_up(g.N, GBN), div_up(g.M, GBM), batch);
     if (radon_precision() == 1) gemm_bf16x3_kernel<<<grid, 256, 0, st>>>(g);
-    else gemm_f32_kernel<<<grid, 256, 0, st>>>(g);
+    else if (g.K < 512) gemm_f32_kernel<<<grid, 256, 0, st>>>(g);
+    else if (g.sAm == 1 && g.sBn == 1) gemm_f32_db<true, true><<<grid, 256, 0, st>>>(g);
+    else if (g.sAm == 1) gemm_f32_db<true, false><<<grid, 256, 0, st>>>(g);
+    else if (g.sBn == 1) gemm_f32_db<false, true><<<grid, 256, 0, st>>>(g);
+    else gemm_f32_db<false, false><<<grid, 256, 0, st>>>(g);
     SWD_HIP(hipGetLastError());
     return kOk;
 }
