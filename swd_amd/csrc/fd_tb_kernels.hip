@@ -70,8 +70,11 @@ struct CoefRow {
 
 // One row of state and coefficients for this lane.  Rows above/below the
 // grid are clamped onto the first/last row, which (like the first/last
-// column) is the zero ring of every state and coefficient plane.
-template <typename T>
+// column) is the zero ring of every state and coefficient plane.  B1: the
+// b1 plane differs from b and is loaded; otherwise (the wrappers set
+// b1 = b, core/staggerfd_eigen.m:106) b1 is b, one load and NT+1 ring
+// registers fewer.
+template <typename T, bool B1>
 struct RowLoad {
     T xx, zz, xz, uu, ww;
     CoefRow<T> c;
@@ -83,7 +86,8 @@ struct RowLoad {
         const int rc = min(max(row, 0), nzbc - 1);
         const unsigned o = (unsigned)rc * rowb + xb;
         xx = gld(xx0, o); zz = gld(zz0, o); xz = gld(xz0, o); uu = gld(uu0, o); ww = gld(ww0, o);
-        c = CoefRow<T>{gld(ct, o), gld(cb, o), gld(cb1, o), gld(cca, o), gld(ccl, o), gld(ccm1, o)};
+        const T b = gld(cb, o);
+        c = CoefRow<T>{gld(ct, o), b, B1 ? gld(cb1, o) : b, gld(cca, o), gld(ccl, o), gld(ccm1, o)};
     }
 };
 
@@ -96,7 +100,7 @@ struct ImgRow {
     T m0, m1, m2;
 };
 
-template <typename T, int NT>
+template <typename T, int NT, bool B1>
 __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJob<T>* __restrict__ jobs,
                                                                       const BlockMap* __restrict__ bmap, int step,
                                                                       int launch)
@@ -172,7 +176,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
     const int i0 = z0 - NT - 1, i1 = z1 - 1 + NT;
     // row loads are issued one iteration ahead so their latency overlaps the
     // level updates of the current row
-    auto row = [&](const RowLoad<T>& cur, const int i) {
+    auto row = [&](const RowLoad<T, B1>& cur, const int i) {
         // level 0: row i of the stored state and its coefficients
         T ixx = cur.xx, izz = cur.zz, ixz = cur.xz, iu = cur.uu, iw = cur.ww;
 #pragma unroll
@@ -247,7 +251,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
     };
     // Rows are loaded one step ahead into two alternating buffers, so the
     // loads of row i+1 are in flight while row i is processed.
-    RowLoad<T> A, B;
+    RowLoad<T, B1> A, B;
     A.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i0, nzbc, rowb, xb);
     for (int i = i0; i <= i1; i += 2) {
         B.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i + 1, nzbc, rowb, xb);
@@ -260,7 +264,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
 
 // Adjoint (reverse-time) counterpart with the imaging condition.  Level l
 // runs adjoint step it = nt - 2 - (k0 + l).
-template <typename T, int NT>
+template <typename T, int NT, bool B1>
 __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJob<T>* __restrict__ jobs,
                                                                       const BlockMap* __restrict__ bmap, int step,
                                                                       int launch)
@@ -370,7 +374,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
     for (int k = 0; k <= NT; ++k) ring[k] = CoefRow<T>{T(0), T(0), T(0), T(0), T(0), T(0)};
 
     const int i0 = z0 - NT - 1, i1 = z1 - 1 + NT;
-    auto row = [&](const RowLoad<T>& cur, const ImgRow<T>& icur, const int i) {
+    auto row = [&](const RowLoad<T, B1>& cur, const ImgRow<T>& icur, const int i) {
         T ixx = cur.xx, izz = cur.zz, ixz = cur.xz, iu = cur.uu, iw = cur.ww;
 #pragma unroll
         for (int k = NT; k > 0; --k) ring[k] = ring[k - 1];
@@ -484,7 +488,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
     };
     // Rows are loaded one step ahead into two alternating buffers, so the
     // loads of row i+1 are in flight while row i is processed.
-    RowLoad<T> A, B;
+    RowLoad<T, B1> A, B;
     ImgRow<T> IA, IB;
     A.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i0, nzbc, rowb, xb);
     img_load(IA, i0);
@@ -499,53 +503,73 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
     }
 }
 
-template <typename T>
-int launch_fwd_tb(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step, int launch,
-                  int nt_block)
+template <typename T, bool B1>
+static int fwd_tb_nt(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step, int launch,
+                     int nt_block)
 {
     const dim3 blk(kMarchLanes * kMarchWaves);
     switch (nt_block) {
-    case 2: fd_fwd_tb<T, 2><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
-    case 3: fd_fwd_tb<T, 3><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
-    case 4: fd_fwd_tb<T, 4><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
-    case 5: fd_fwd_tb<T, 5><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
-    case 6: fd_fwd_tb<T, 6><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    case 2: fd_fwd_tb<T, 2, B1><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    case 3: fd_fwd_tb<T, 3, B1><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    case 4: fd_fwd_tb<T, 4, B1><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    case 5: fd_fwd_tb<T, 5, B1><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    case 6: fd_fwd_tb<T, 6, B1><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
     default: set_error("unsupported time block %d", nt_block); return kErrArg;
     }
     SWD_HIP(hipGetLastError());
     return kOk;
+}
+
+template <typename T, bool B1>
+static int adj_tb_nt(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step, int launch,
+                     int nt_block)
+{
+    const dim3 blk(kMarchLanes * kMarchWaves);
+    switch (nt_block) {
+    case 2: fd_adj_tb<T, 2, B1><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    case 3: fd_adj_tb<T, 3, B1><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    case 4: fd_adj_tb<T, 4, B1><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    case 5: fd_adj_tb<T, 5, B1><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    case 6: fd_adj_tb<T, 6, B1><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
+    default: set_error("unsupported time block %d", nt_block); return kErrArg;
+    }
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
+template <typename T>
+int launch_fwd_tb(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step, int launch,
+                  int nt_block, bool b1)
+{
+    return b1 ? fwd_tb_nt<T, true>(st, d_jobs, d_bmap, nblocks, step, launch, nt_block)
+              : fwd_tb_nt<T, false>(st, d_jobs, d_bmap, nblocks, step, launch, nt_block);
 }
 
 template <typename T>
 int launch_adj_tb(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step, int launch,
-                  int nt_block)
+                  int nt_block, bool b1)
 {
-    const dim3 blk(kMarchLanes * kMarchWaves);
-    switch (nt_block) {
-    case 2: fd_adj_tb<T, 2><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
-    case 3: fd_adj_tb<T, 3><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
-    case 4: fd_adj_tb<T, 4><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
-    case 5: fd_adj_tb<T, 5><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
-    case 6: fd_adj_tb<T, 6><<<nblocks, blk, 0, st>>>(d_jobs, d_bmap, step, launch); break;
-    default: set_error("unsupported time block %d", nt_block); return kErrArg;
-    }
-    SWD_HIP(hipGetLastError());
-    return kOk;
+    return b1 ? adj_tb_nt<T, true>(st, d_jobs, d_bmap, nblocks, step, launch, nt_block)
+              : adj_tb_nt<T, false>(st, d_jobs, d_bmap, nblocks, step, launch, nt_block);
 }
 
 // Resident workgroups per CU of the time-blocked kernel (occupancy API), 0 if unknown.
 template <typename T>
-int tb_blocks_per_cu(bool adj, int nt_block)
+int tb_blocks_per_cu(bool adj, int nt_block, bool b1)
 {
     const void* f = nullptr;
+#define SWD_TB_FN(N)                                                                                              \
+    f = adj ? (b1 ? (const void*)fd_adj_tb<T, N, true> : (const void*)fd_adj_tb<T, N, false>)                    \
+            : (b1 ? (const void*)fd_fwd_tb<T, N, true> : (const void*)fd_fwd_tb<T, N, false>)
     switch (nt_block) {
-    case 2: f = adj ? (const void*)fd_adj_tb<T, 2> : (const void*)fd_fwd_tb<T, 2>; break;
-    case 3: f = adj ? (const void*)fd_adj_tb<T, 3> : (const void*)fd_fwd_tb<T, 3>; break;
-    case 4: f = adj ? (const void*)fd_adj_tb<T, 4> : (const void*)fd_fwd_tb<T, 4>; break;
-    case 5: f = adj ? (const void*)fd_adj_tb<T, 5> : (const void*)fd_fwd_tb<T, 5>; break;
-    case 6: f = adj ? (const void*)fd_adj_tb<T, 6> : (const void*)fd_fwd_tb<T, 6>; break;
+    case 2: SWD_TB_FN(2); break;
+    case 3: SWD_TB_FN(3); break;
+    case 4: SWD_TB_FN(4); break;
+    case 5: SWD_TB_FN(5); break;
+    case 6: SWD_TB_FN(6); break;
     default: return 0;
     }
+#undef SWD_TB_FN
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kMarchLanes * kMarchWaves, 0) != hipSuccess) {
         (void)hipGetLastError();
@@ -553,12 +577,12 @@ int tb_blocks_per_cu(bool adj, int nt_block)
     }
     return nb;
 }
-template int tb_blocks_per_cu<float>(bool, int);
-template int tb_blocks_per_cu<double>(bool, int);
+template int tb_blocks_per_cu<float>(bool, int, bool);
+template int tb_blocks_per_cu<double>(bool, int, bool);
 
-template int launch_fwd_tb<float>(hipStream_t, const FdJob<float>*, const BlockMap*, int, int, int, int);
-template int launch_adj_tb<float>(hipStream_t, const FdJob<float>*, const BlockMap*, int, int, int, int);
-template int launch_fwd_tb<double>(hipStream_t, const FdJob<double>*, const BlockMap*, int, int, int, int);
-template int launch_adj_tb<double>(hipStream_t, const FdJob<double>*, const BlockMap*, int, int, int, int);
+template int launch_fwd_tb<float>(hipStream_t, const FdJob<float>*, const BlockMap*, int, int, int, int, bool);
+template int launch_adj_tb<float>(hipStream_t, const FdJob<float>*, const BlockMap*, int, int, int, int, bool);
+template int launch_fwd_tb<double>(hipStream_t, const FdJob<double>*, const BlockMap*, int, int, int, int, bool);
+template int launch_adj_tb<double>(hipStream_t, const FdJob<double>*, const BlockMap*, int, int, int, int, bool);
 
 }  // namespace swd

@@ -162,6 +162,7 @@ int Engine::set_model(const double* vp, const double* vs)
         pack_grid(tmp.data(), nzbc, nxbc, pitch, h.data() + (size_t)k * plane);
         zero_ring(h.data() + (size_t)k * plane, nzbc, nxbc, pitch);
     }
+    b1_same = std::equal(h.begin() + plane, h.begin() + 2 * plane, h.begin() + 2 * plane);
     SWD_HIP(hipMemcpyAsync(coefs.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, st));
     SWD_HIP(hipMemcpyAsync(vs_d.p, vs, splane * sizeof(double), hipMemcpyHostToDevice, st));
     SWD_HIP(hipMemcpyAsync(vp_d.p, vp, splane * sizeof(double), hipMemcpyHostToDevice, st));
@@ -189,6 +190,7 @@ int Engine::set_vs_device(const double* vs_dev)
     if ((rc = launch_prep_coefs(st, vp_d.as<double>(), vs_dev, geo.as<double>(), kappa, p.nz, p.nx, p.nbc, pt_m, nzbc,
                                 nxbc, pitch, plane, p.dt, p.dt / p.dx, coefs.as<float>())))
         return rc;
+    b1_same = true;  // prep_coefs_kernel writes b1 = b
     if (vs_dev != vs_d.as<double>())
         SWD_HIP(hipMemcpyAsync(vs_d.p, vs_dev, splane * sizeof(double), hipMemcpyDeviceToDevice, st));
     SWD_HIP(hipStreamSynchronize(st));
@@ -320,8 +322,8 @@ void Engine::make_jobs(std::vector<FdJob<float>>& jobs, bool adj, bool snapshots
             J.zz[b] = S + (5 * b + 3) * plane;
             J.xz[b] = S + (5 * b + 4) * plane;
         }
-        J.temp = C; J.b = C + plane; J.b1 = C + 2 * plane; J.ca = C + 3 * plane; J.cl = C + 4 * plane;
-        J.cm1 = C + 5 * plane;
+        J.temp = C; J.b = C + plane; J.b1 = b1_same ? J.b : C + 2 * plane; J.ca = C + 3 * plane;
+        J.cl = C + 4 * plane; J.cm1 = C + 5 * plane;
         const size_t ntr = (size_t)p.nt * p.ng;
         J.src = adj ? adjsrc.as<float>() + s * ntr : wavelet.as<float>();
         J.trace_w = adj ? nullptr : traces.as<float>() + s * ntr;

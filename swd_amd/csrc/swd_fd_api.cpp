@@ -3,6 +3,8 @@
 // Ricker wavelet.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <atomic>
 #include <cmath>
 #include <cstdarg>
@@ -183,6 +185,8 @@ static int staggerfd_impl(const T* iv, const T* const coef_in[7], const T* s, T*
         J.xz[b] = S + (5 * b + 4) * plane;
     }
     J.temp = C; J.b = C + plane; J.b1 = C + 2 * plane; J.ca = C + 3 * plane; J.cl = C + 4 * plane; J.cm1 = C + 5 * plane;
+    // b1 equal to b (as the wrappers make it): the fused kernels then read b for both
+    if (fused && std::equal(hc.begin() + plane, hc.begin() + 2 * plane, hc.begin() + 2 * plane)) J.b1 = J.b;
     J.src = (const T*)src.p;
     J.trace_w = (T*)traces.p;
     J.trace_u = seismo_u ? (T*)traces.p + ntr : nullptr;
@@ -326,6 +330,8 @@ static int e2drtm_impl(const T* iv, const T* const coef_in[7], const T* adj, con
         J.xz[b] = S + (5 * b + 4) * plane;
     }
     J.temp = C; J.b = C + plane; J.b1 = C + 2 * plane; J.ca = C + 3 * plane; J.cl = C + 4 * plane; J.cm1 = C + 5 * plane;
+    // b1 equal to b (as the wrappers make it): the fused kernels then read b for both
+    if (fused && std::equal(hc.begin() + plane, hc.begin() + 2 * plane, hc.begin() + 2 * plane)) J.b1 = J.b;
     J.src = (const T*)dadj.p;
     J.snap = (T*)snaps.p;
     J.img = (double*)img.p;

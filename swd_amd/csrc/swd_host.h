@@ -173,15 +173,26 @@ int launch_fwd_march(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_b
 template <typename T>
 int launch_adj_march(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step,
                      int launch);
+// b1: some job's b1 plane differs from its b plane (jobs whose planes are
+// equal point b1 at b, see jobs_b1)
 template <typename T>
 int launch_fwd_tb(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step, int launch,
-                  int nt_block);
+                  int nt_block, bool b1);
 template <typename T>
 int launch_adj_tb(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step, int launch,
-                  int nt_block);
+                  int nt_block, bool b1);
 
 template <typename T>
-int tb_blocks_per_cu(bool adjoint, int nt_block);
+int tb_blocks_per_cu(bool adjoint, int nt_block, bool b1 = false);
+
+// whether any job reads a b1 plane of its own (J.b1 != J.b)
+template <typename T>
+inline bool jobs_b1(const std::vector<FdJob<T>>& jobs)
+{
+    for (const auto& J : jobs)
+        if (J.b1 != J.b) return true;
+    return false;
+}
 
 // Time-blocking settings: steps per launch (1 = single-step kernels) and band
 // rows of the time-blocked kernels (0 = chosen from occupancy); process-wide

@@ -51,7 +51,7 @@ int auto_band_rows(const std::vector<FdJob<T>>& jobs, bool adj, int NT)
         (void)hipGetLastError();
         cus = 0;
     }
-    const int nb = tb_blocks_per_cu<T>(adj, NT);
+    const int nb = tb_blocks_per_cu<T>(adj, NT, jobs_b1(jobs));
     const long long slots = (long long)std::max(cus, 1) * std::max(nb, 1) * kMarchWaves;
     const int out_cols = kMarchLanes - 2 * NT;
     int zmax = 1;
@@ -82,6 +82,7 @@ int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int steps, 
         if (J.start != 0 || J.launch0 != 0) { set_error("run_march jobs must start together"); return kErrArg; }
     }
     const int NT = tb_ok ? nt_block : 1;
+    const bool b1 = jobs_b1(jobs);
     const int nL = NT > 1 ? steps / NT : 0;
     const int rem = steps - nL * NT;
     int rc;
@@ -114,8 +115,8 @@ int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int steps, 
     for (int L = 0; L < nL && rc == kOk; ++L, ++launch) {
         const bool samp = L % stride == 0;
         if (samp) SWD_HIP(hipEventRecord(ev[2 * (L / stride)], st));
-        rc = adj ? launch_adj_tb<T>(st, djt.as<FdJob<T>>(), dbt.as<BlockMap>(), ptb.nblocks, L * NT, launch, NT)
-                 : launch_fwd_tb<T>(st, djt.as<FdJob<T>>(), dbt.as<BlockMap>(), ptb.nblocks, L * NT, launch, NT);
+        rc = adj ? launch_adj_tb<T>(st, djt.as<FdJob<T>>(), dbt.as<BlockMap>(), ptb.nblocks, L * NT, launch, NT, b1)
+                 : launch_fwd_tb<T>(st, djt.as<FdJob<T>>(), dbt.as<BlockMap>(), ptb.nblocks, L * NT, launch, NT, b1);
         if (samp && rc == kOk) SWD_HIP(hipEventRecord(ev[2 * (L / stride) + 1], st));
     }
     for (int j = 0; j < rem && rc == kOk; ++j, ++launch) {

@@ -221,6 +221,7 @@ struct Server {
     std::vector<Request*> table_of[2];  // members in the current launch table
     DevBuf djobs[2], dbmap[2];
     int nblocks[2] = {0, 0};
+    bool b1_tab[2] = {false, false};  // the launch table has a job with its own b1 plane
     char* stage[2][kStage] = {};
     hipEvent_t stage_ev[2][kStage] = {};
     bool stage_used[2][kStage] = {};
@@ -612,6 +613,7 @@ int Server::rebuild(int ph)
     SWD_HIP(hipEventRecord(stage_ev[ph][k], fd[ph]));
     stage_used[ph][k] = true;
     nblocks[ph] = plan.nblocks;
+    b1_tab[ph] = jobs_b1(jobs);
     return kOk;
 }
 
@@ -695,9 +697,9 @@ void Server::loop()
             for (int ph = 0; ph < 2 && rc == kOk; ++ph) {
                 if (table_of[ph].empty()) continue;
                 rc = ph == 0 ? launch_fwd_tb<float>(fd[0], djobs[0].as<FdJob<float>>(), dbmap[0].as<BlockMap>(),
-                                                    nblocks[0], gstep[0], glaunch[0], nt_blk[0])
+                                                    nblocks[0], gstep[0], glaunch[0], nt_blk[0], b1_tab[0])
                              : launch_adj_tb<float>(fd[1], djobs[1].as<FdJob<float>>(), dbmap[1].as<BlockMap>(),
-                                                    nblocks[1], gstep[1], glaunch[1], nt_blk[1]);
+                                                    nblocks[1], gstep[1], glaunch[1], nt_blk[1], b1_tab[1]);
                 gstep[ph] += nt_blk[ph];
                 ++glaunch[ph];
                 ++n_launch;

@@ -93,3 +93,55 @@ def test_engine_gradient_bitwise_across_time_blocks():
     np.testing.assert_array_equal(g["cr_r"], g1["cr_r"])
     np.testing.assert_array_equal(g["dk_vs"], g1["dk_vs"])
     e.close()
+
+
+def test_mex_separate_b1_plane(case):
+    """The MEX entry points take b1 separately from b.  With b1 != b the
+    time-blocked kernels read the b1 plane: time blocks stay bitwise equal to
+    the single-step kernels, the result follows the FP64 oracle with the
+    same planes, and it differs from the b1 = b result."""
+    a = case
+    is_, nbc, nt, dtx, dx, dt, sx, sz, gx, gz, s, vp, vs = a[:13]
+    c = dict(fd.prepare_model(vp, vs, nbc, dx, dt, 22))
+    c_b1 = dict(c)
+    c_b1["b1"] = c["b1"] * np.linspace(0.95, 1.05, c["b1"].shape[1])[None, :]
+    iv = fd._fwd_iv(nt, vp.shape, nbc, dtx, gx, gz, sx, sz, "w", 22, 5)
+
+    def run(cc, nt_block):
+        fd.set_time_block(nt_block, 0)
+        try:
+            return fd.eigen_staggerfd_single(iv, cc["temp"], cc["ca"], cc["cl"], cc["cm"], cc["cm1"], cc["b"],
+                                             cc["b1"], s)
+        finally:
+            fd.set_time_block(4, 0)
+
+    ref1 = run(c_b1, 1)
+    for nt_block in (4, 5):
+        out = run(c_b1, nt_block)
+        np.testing.assert_array_equal(out[1], ref1[1])
+        for k in ref1[2]:
+            np.testing.assert_array_equal(out[2][k], ref1[2][k])
+    same = run(c, 4)
+    _, sw64, _ = O.staggerfd(iv, c_b1, s, "d", snapshots=False)
+    e = float(np.linalg.norm(ref1[1] - sw64) / np.linalg.norm(sw64))
+    d = float(np.linalg.norm(same[1] - ref1[1]) / np.linalg.norm(ref1[1]))
+    print(f"separate b1: rel-L2 vs FP64 {e:.2e}; differs from b1 = b by {d:.2e}")
+    assert e < 1e-4 and d > 1e-3
+    # adjoint + imaging with the same planes: time blocks bitwise equal to single steps
+    nz, nx = vp.shape
+    iva = np.array([nt, nz + 2 + nbc, nx + 2 * nbc, dtx, gx.size, nbc, 1 + 2, gx[0] + nbc, 1.0, 22, 5, nz, nx, dt])
+    adj = np.random.default_rng(4).standard_normal((nt, gx.size)).astype(np.float32)
+    S = ref1[2]
+
+    def run_adj(nt_block):
+        fd.set_time_block(nt_block, 0)
+        try:
+            return fd.eigen_e2drtm_single(iva, c_b1["temp"], c_b1["ca"], c_b1["cl"], c_b1["cm"], c_b1["cm1"],
+                                          c_b1["b"], c_b1["b1"], adj, S["fux"], S["fuz"], S["bwx"], S["bwz"])
+        finally:
+            fd.set_time_block(4, 0)
+
+    g1 = run_adj(1)
+    for nt_block in (4, 5):
+        for x, y in zip(run_adj(nt_block), g1):
+            np.testing.assert_array_equal(x, y)
