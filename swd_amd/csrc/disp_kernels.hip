@@ -721,9 +721,28 @@ __global__ __launch_bounds__(64) void pick_kernel(const float* ml, const float* 
     __shared__ int idx[kMaxPair], dvmax[kMaxPair], dvmin[kMaxPair], chz[kMaxPair];
     __shared__ float qj[kMaxPair];
     __shared__ double fidx[kMaxPair];
+    // the frequency column being tracked and the next one, staged in LDS: the
+    // next column's loads are in flight while the current one is searched
+    __shared__ float colv[2][kPickMaxNp];
     if (!c.enabled) return;
     const float* base = ml + (size_t)col * npair * np;
     const int a = np;
+    const int nk = (np + 63) >> 6;
+    float pf[kPickMaxNp / 64];
+    auto fetch = [&](int i) {
+#pragma unroll
+        for (int k = 0; k < kPickMaxNp / 64; ++k) {
+            const int j = k * 64 + lane;
+            if (k < nk) pf[k] = j < np ? base[(size_t)i * np + j] : 0.f;
+        }
+    };
+    auto put = [&](int b) {
+#pragma unroll
+        for (int k = 0; k < kPickMaxNp / 64; ++k) {
+            const int j = k * 64 + lane;
+            if (k < nk && j < np) colv[b][j] = pf[k];
+        }
+    };
 
     // argmax over v[lo..hi] (NaN ignored, first index of the maximum); 1-based result
     auto argmax = [&](const float* v, int lo, int hi) -> int {
@@ -778,38 +797,54 @@ __global__ __launch_bounds__(64) void pick_kernel(const float* ml, const float* 
     }
     const int L1 = lp - 1;  // 0-based index of lp
     if (lane == 0) for (int i = 0; i < npair; ++i) { chz[i] = 1; qj[i] = NAN; dvmax[i] = 0; dvmin[i] = 0; }
+    fetch(L1);
+    put(0);
     __syncthreads();
-    int cur = argmax(base + (size_t)L1 * np, 0, a - 1);
+    int cur = argmax(colv[0], 0, a - 1);
     if (lane == 0) idx[L1] = cur;
+    if (L1 >= 1) {
+        fetch(L1 - 1);
+        put(1);
+    }
     __syncthreads();
-    // left of lp
+    // left of lp (column i in colv[b])
+    int b = 1;
     for (int i = L1 - 1; i >= 0; --i) {
-        const float* v = base + (size_t)i * np;
+        if (i >= 1) fetch(i - 1);
+        const float* v = colv[b];
         const int ip = idx[i + 1];
         if (ip == a) {
             if (lane == 0) { idx[i] = a; dvmax[i] = dvmax[i + 1]; dvmin[i] = dvmin[i + 1]; }
-            __syncthreads();
-            continue;
-        }
-        int dmax, dmin;
-        if (ip > a - 3) dmax = a;
-        else { const int f = first_min(v, ip - 1, a - 1); dmax = f ? ip + f : a; }
-        if (ip < 3) dmin = 1;
-        else { const int f = last_min(v, 0, ip - 1); dmin = f ? f : 1; }
-        const float q = (float)(dmax - dmin);
-        const float qprev = qj[i + 1];
-        int newidx;
-        if (q > qprev * lfg) {
-            if (lane == 0) { chz[i] = 0; dvmax[i] = dvmax[i + 1]; dvmin[i] = dvmin[i + 1]; qj[i] = (float)(dvmax[i + 1] - dvmin[i + 1]); idx[i] = ip; }
         } else {
-            newidx = dmin + argmax(v, dmin - 1, dmax - 1) - 1;
-            if (lane == 0) { idx[i] = newidx; dvmax[i] = dmax; dvmin[i] = dmin; qj[i] = q; }
+            int dmax, dmin;
+            if (ip > a - 3) dmax = a;
+            else { const int f = first_min(v, ip - 1, a - 1); dmax = f ? ip + f : a; }
+            if (ip < 3) dmin = 1;
+            else { const int f = last_min(v, 0, ip - 1); dmin = f ? f : 1; }
+            const float q = (float)(dmax - dmin);
+            const float qprev = qj[i + 1];
+            if (q > qprev * lfg) {
+                if (lane == 0) { chz[i] = 0; dvmax[i] = dvmax[i + 1]; dvmin[i] = dvmin[i + 1]; qj[i] = (float)(dvmax[i + 1] - dvmin[i + 1]); idx[i] = ip; }
+            } else {
+                const int newidx = dmin + argmax(v, dmin - 1, dmax - 1) - 1;
+                if (lane == 0) { idx[i] = newidx; dvmax[i] = dmax; dvmin[i] = dmin; qj[i] = q; }
+            }
         }
         __syncthreads();
+        if (i >= 1) put(b ^ 1);
+        __syncthreads();
+        b ^= 1;
     }
     // right of lp
+    if (L1 + 1 < npair) {
+        fetch(L1 + 1);
+        put(0);
+    }
+    __syncthreads();
+    b = 0;
     for (int i = L1 + 1; i < npair; ++i) {
-        const float* v = base + (size_t)i * np;
+        if (i + 1 < npair) fetch(i + 1);
+        const float* v = colv[b];
         const int ip = idx[i - 1];
         int dmax, dmin;
         if (ip > a - 3) dmax = a;
@@ -825,6 +860,9 @@ __global__ __launch_bounds__(64) void pick_kernel(const float* ml, const float* 
             if (lane == 0) { idx[i] = newidx; dvmax[i] = dmax; dvmin[i] = dmin; qj[i] = q; }
         }
         __syncthreads();
+        if (i + 1 < npair) put(b ^ 1);
+        __syncthreads();
+        b ^= 1;
     }
     // interp1(que, idx(que), 1:b, 'linear') then nearest extrapolation
     if (lane == 0) {

@@ -8,6 +8,7 @@
 namespace swd {
 
 constexpr int kMaxPair = 1024;  // frequency columns a pick kernel can hold in LDS
+constexpr int kPickMaxNp = 2048;  // velocities of a column the pick kernel stages in LDS
 
 // One (shot, side) spectrum column.
 struct DispCol {

@@ -37,6 +37,7 @@ int disp_geometry(DispGeom& g)
     g.nt_eff = std::min(g.nt, g.ccn);
     if (g.lf < 1 || g.npair < 1 || g.np < 1 || g.ccn < 1 || g.nf > g.ccn) { set_error("bad Radon frequency/velocity axes"); return kErrArg; }
     if (g.npair > kMaxPair) { set_error("npair %d exceeds %d", g.npair, kMaxPair); return kErrArg; }
+    if (g.np > kPickMaxNp) { set_error("np %d exceeds %d", g.np, kPickMaxNp); return kErrArg; }
     if (!(g.vmin > 0)) { set_error("vmin must be positive"); return kErrArg; }
     return kOk;
 }
@@ -251,6 +252,7 @@ static int lhdisppick_impl(const float* ml, int np, int npair, double vmin, cons
 {
     if (!ml || !cr_0 || !cr_pre || np < 1 || npair < 1) { set_error("bad LHDispPick arguments"); return kErrArg; }
     if (npair > kMaxPair) { set_error("npair %d exceeds %d", npair, kMaxPair); return kErrArg; }
+    if (np > kPickMaxNp) { set_error("np %d exceeds %d", np, kPickMaxNp); return kErrArg; }
     int rc;
     Stream st;
     if ((rc = st.create())) return rc;
