@@ -361,6 +361,7 @@ def main():
     ap.add_argument("--time-block", type=int, default=0,
                     help="FD steps per launch (1: single-step kernels; 0: chosen by the autotuner)")
     ap.add_argument("--band-rows", type=int, default=0, help="band rows of the time-blocked kernels (0: automatic)")
+    ap.add_argument("--no-graph", action="store_true", help="launch the FD phases eagerly instead of replaying HIP graphs")
     ap.add_argument("--plain-shots", type=int, default=2, help="shots for the unfused comparator (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven multi-model batch leg")
@@ -393,6 +394,7 @@ def main():
         tune = eng.autotune()
         tuned = {"fwd": list(tune["fwd"]), "adj": list(tune["adj"]), "candidates": tune["candidates"],
                  "seconds": round(time.perf_counter() - t0, 2)}
+    eng.set_graph(not args.no_graph)       # FD phases replay HIP graphs, captured at the first gradient
     for _ in range(args.warmup):
         eng.gradient()
     per_step, fwd_us, adj_us, stage = [], [], [], []
@@ -451,7 +453,7 @@ def main():
                         for k in ("fwd_ms", "disp_ms", "adj_ms", "asm_ms")},
            "roofline": roof,
            "fd_launch": {"fwd": [kf, int(stage[-1]["fwd_band_rows"])], "adj": [ka, int(stage[-1]["adj_band_rows"])],
-                         "autotune": tuned},
+                         "autotune": tuned, "hip_graph": not args.no_graph},
            "device_bytes": info["device_bytes"],
            "residual_r_mean": float(np.mean(g["res_r"]))}
 

@@ -56,6 +56,7 @@ SIGNATURES = {
     "swd_engine_gradient": (_I, [_P, _P, _P, _P, _P, _P]),
     "swd_engine_forward_picks": (_I, [_P, _P, _P, _P, _P]),
     "swd_engine_set_fd_path": (_I, [_P, _I]),
+    "swd_engine_set_graph": (_I, [_P, _I]),
     "swd_engine_set_variant": (_I, [_P, _P]),
     "swd_engine_run": (_I, [_P, _I]),
     "swd_engine_set_picks": (_I, [_P, _P, _P]),

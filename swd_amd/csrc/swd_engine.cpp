@@ -369,6 +369,26 @@ int run_fd_batch(const std::vector<Engine*>& engines, bool adj, bool snapshots, 
 
 int Engine::run_fd(bool adj, bool snapshots)
 {
+    if (fd_path == 0 && use_graph) {
+        // zero the state and outputs on the stream, then replay (or capture) the launches
+        std::vector<FdJob<float>> jobs;
+        make_jobs(jobs, adj, snapshots);
+        SWD_HIP(hipMemsetAsync(state.p, 0, state.bytes, st));
+        if (adj) {
+            SWD_HIP(hipMemsetAsync(imgs.p, 0, imgs.bytes, st));
+            SWD_HIP(hipMemsetAsync(imgp.p, 0, imgp.bytes, st));
+        } else {
+            SWD_HIP(hipMemsetAsync(traces.p, 0, traces.bytes, st));
+        }
+        MarchStats ms;
+        const int rc = run_march_graph<float>(st, jobs, adj, adj ? p.nt - 1 : p.nt, nt_block[adj], tb_rows[adj], tz,
+                                              graphs[adj][snapshots], &ms);
+        if (rc == kOk) {
+            if (adj) { times.adj_kernel_us = ms.kernel_us; times.adj_spl = ms.nt_block; times.adj_rows = ms.rows; }
+            else { times.fwd_kernel_us = ms.kernel_us; times.fwd_spl = ms.nt_block; times.fwd_rows = ms.rows; }
+        }
+        return rc;
+    }
     if (fd_path == 0) {
         MarchStats ms;
         const int rc = run_fd_batch({this}, adj, snapshots, st, tz, &ms);
@@ -599,6 +619,14 @@ int swd_engine_set_variant(swd_engine* e, const swd_variant* v)
     if ((rc = set_plan_variant(g.disp, *v))) return rc;
     g.vp_ratio = v->vp_ratio;
     return kOk;
+}
+
+int swd_engine_set_graph(swd_engine* e, int on)
+{
+    if (!e) return kErrArg;
+    const int old = e->e.use_graph ? 1 : 0;
+    e->e.use_graph = on != 0;
+    return old;
 }
 
 int swd_engine_set_fd_path(swd_engine* e, int mode)

@@ -43,8 +43,10 @@ public:
     int tb_rows[2] = {0, 0};   // band rows of the time-blocked march, forward / adjoint (0: from occupancy)
     bool have_model = false, have_wavelet = false;
     double vpmax = 0;
-    double vp_ratio = 0;
-    bool b1_same = false;  // the b1 plane equals b (jobs then read b for b1)  // > 0: device model updates set vp = vp_ratio * vs (field scripts); 0: vp fixed (SWD.m)
+    double vp_ratio = 0;   // > 0: device model updates set vp = vp_ratio * vs (field scripts); 0: vp fixed (SWD.m)
+    bool b1_same = false;  // the b1 plane equals b (jobs then read b for b1)
+    bool use_graph = false;    // FD phases replay captured HIP graphs
+    MarchGraph graphs[2][2];   // [adjoint][snapshots]
 
     DevBuf coefs;     // 6 planes float: temp, b, b1, ca, cl, cm1
     DevBuf vs_d;      // nz x nx double (MATLAB layout) for the assembly

@@ -220,6 +220,48 @@ struct MarchStats {
 template <typename T>
 int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adjoint, int steps, int nt_block, int tb_rows,
               int ss_rows, MarchStats* stats);
+
+// The same march as captured HIP graphs (the time-blocked launches, then the
+// single-step remainder), replayed while the jobs and the launch parameters
+// stay the same (model, wavelet and adjoint-source contents may change
+// between replays; pointers and sizes may not).  The graphs hold the launches
+// only: whatever zeroes the state runs before them on the stream.
+// stats->kernel_us is the dominant graph's event-timed span over its launch
+// count (so it includes the gaps between its kernels).
+struct MarchGraph {
+    std::vector<unsigned char> key;  // launch parameters and job tables it was captured for
+    DevBuf djt, dbt, djs, dbs;       // the job tables and block maps its kernels read
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    int launches[2] = {0, 0};
+    int nt_block = 1, rows = 0;
+    bool ready = false;
+    MarchGraph() = default;
+    MarchGraph(const MarchGraph&) = delete;
+    MarchGraph& operator=(const MarchGraph&) = delete;
+    ~MarchGraph()
+    {
+        reset();
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    void reset()
+    {
+        for (int i = 0; i < 2; ++i) {
+            if (exec[i]) (void)hipGraphExecDestroy(exec[i]);
+            if (graph[i]) (void)hipGraphDestroy(graph[i]);
+            exec[i] = nullptr;
+            graph[i] = nullptr;
+            launches[i] = 0;
+        }
+        ready = false;
+        key.clear();
+    }
+};
+template <typename T>
+int run_march_graph(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adjoint, int steps, int nt_block, int tb_rows,
+                    int ss_rows, MarchGraph& g, MarchStats* stats);
 template <typename T>
 int plain_forward_step(hipStream_t st, const FdJob<T>& J, T* const w[4], int it);
 template <typename T>

@@ -149,6 +149,105 @@ int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int steps, 
     return kOk;
 }
 template int run_march<float>(hipStream_t, std::vector<FdJob<float>>&, bool, int, int, int, int, MarchStats*);
+
+template <typename T>
+static void key_add(std::vector<unsigned char>& k, const T& v)
+{
+    const unsigned char* p = (const unsigned char*)&v;
+    k.insert(k.end(), p, p + sizeof(T));
+}
+
+template <typename T>
+int run_march_graph(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int steps, int nt_block, int tb_rows,
+                    int ss_rows, MarchGraph& g, MarchStats* stats)
+{
+    bool tb_ok = nt_block >= 2;
+    for (const auto& J : jobs) {
+        if (adj && std::is_same<T, float>::value && !J.imgp) { set_error("float adjoint jobs need partial images"); return kErrArg; }
+        if (J.stype != 0 && J.stype != 5) tb_ok = false;
+        if (J.start != 0 || J.launch0 != 0) { set_error("run_march jobs must start together"); return kErrArg; }
+    }
+    const int NT = tb_ok ? nt_block : 1;
+    const bool b1 = jobs_b1(jobs);
+    const int nL = NT > 1 ? steps / NT : 0;
+    const int rem = steps - nL * NT;
+    int rc;
+    if (nL > 0 && tb_rows <= 0) tb_rows = auto_band_rows(jobs, adj, NT);
+    std::vector<FdJob<T>> jt = jobs, js = jobs;
+    MarchPlan ptb, pss;
+    if (nL > 0 && (rc = build_march_plan(jt, tb_rows, ptb, kMarchLanes - 2 * NT))) return rc;
+    if (rem > 0 && (rc = build_march_plan(js, ss_rows, pss, kMarchOut))) return rc;
+    std::vector<unsigned char> key;
+    key_add(key, adj); key_add(key, steps); key_add(key, NT); key_add(key, tb_rows); key_add(key, ss_rows);
+    key_add(key, b1);
+    const unsigned char* pj = (const unsigned char*)jt.data();
+    key.insert(key.end(), pj, pj + jt.size() * sizeof(FdJob<T>));
+    pj = (const unsigned char*)js.data();
+    key.insert(key.end(), pj, pj + js.size() * sizeof(FdJob<T>));
+    for (auto& e : g.ev)
+        if (!e) SWD_HIP(hipEventCreate(&e));
+    if (!g.ready || key != g.key) {
+        g.reset();
+        // the tables go up before the capture: the graphs' kernels read them at every replay
+        if (nL > 0 && ((rc = g.djt.upload(jt.data(), jt.size() * sizeof(FdJob<T>), st)) ||
+                       (rc = g.dbt.upload(ptb.bmap.data(), ptb.bmap.size() * sizeof(BlockMap), st))))
+            return rc;
+        if (rem > 0 && ((rc = g.djs.upload(js.data(), js.size() * sizeof(FdJob<T>), st)) ||
+                        (rc = g.dbs.upload(pss.bmap.data(), pss.bmap.size() * sizeof(BlockMap), st))))
+            return rc;
+        SWD_HIP(hipStreamSynchronize(st));
+        // part 0: the time-blocked launches, part 1: the single-step remainder
+        for (int part = 0; part < 2; ++part) {
+            const int n = part == 0 ? nL : rem;
+            if (n == 0) continue;
+            SWD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            rc = kOk;
+            for (int i = 0; i < n && rc == kOk; ++i) {
+                if (part == 0)
+                    rc = adj ? launch_adj_tb<T>(st, g.djt.as<FdJob<T>>(), g.dbt.as<BlockMap>(), ptb.nblocks, i * NT, i, NT, b1)
+                             : launch_fwd_tb<T>(st, g.djt.as<FdJob<T>>(), g.dbt.as<BlockMap>(), ptb.nblocks, i * NT, i, NT, b1);
+                else
+                    rc = adj ? launch_adj_march<T>(st, g.djs.as<FdJob<T>>(), g.dbs.as<BlockMap>(), pss.nblocks, nL * NT + i, nL + i)
+                             : launch_fwd_march<T>(st, g.djs.as<FdJob<T>>(), g.dbs.as<BlockMap>(), pss.nblocks, nL * NT + i, nL + i);
+            }
+            hipGraph_t graph = nullptr;
+            const hipError_t ce = hipStreamEndCapture(st, &graph);
+            if (rc != kOk || ce != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                if (rc == kOk) { set_error("graph capture failed: %s", hipGetErrorString(ce)); rc = kErrHip; }
+                g.reset();
+                return rc;
+            }
+            g.graph[part] = graph;
+            SWD_HIP(hipGraphInstantiate(&g.exec[part], graph, nullptr, nullptr, 0));
+            g.launches[part] = n;
+        }
+        g.key.swap(key);
+        g.ready = true;
+        g.nt_block = nL > 0 ? NT : 1;
+        g.rows = nL > 0 ? tb_rows : ss_rows;
+    }
+    SWD_HIP(hipEventRecord(g.ev[0], st));
+    if (g.exec[0]) SWD_HIP(hipGraphLaunch(g.exec[0], st));
+    SWD_HIP(hipEventRecord(g.ev[1], st));
+    if (g.exec[1]) SWD_HIP(hipGraphLaunch(g.exec[1], st));
+    SWD_HIP(hipEventRecord(g.ev[2], st));
+    SWD_HIP(hipStreamSynchronize(st));
+    float span[2] = {0, 0};
+    for (int part = 0; part < 2; ++part)
+        if (hipEventElapsedTime(&span[part], g.ev[part], g.ev[part + 1]) != hipSuccess) span[part] = 0;
+    if (stats) {
+        // launches of the dominant kind: their graph's span over its launch count
+        const int dom = g.launches[0] > 0 ? 0 : 1;
+        stats->span_ms = span[0] + span[1];
+        stats->kernel_us = g.launches[dom] > 0 ? 1000.f * span[dom] / g.launches[dom] : 0.f;
+        stats->nt_block = g.nt_block;
+        stats->rows = g.rows;
+    }
+    return kOk;
+}
+template int run_march_graph<float>(hipStream_t, std::vector<FdJob<float>>&, bool, int, int, int, int, MarchGraph&,
+                                    MarchStats*);
 template int run_march<double>(hipStream_t, std::vector<FdJob<double>>&, bool, int, int, int, int, MarchStats*);
 
 }  // namespace swd

@@ -165,6 +165,10 @@ class Engine:
         check(lib().swd_engine_set_variant(self.h, ctypes.byref(v)))
         self.variant = v
 
+    def set_graph(self, on: bool = True) -> bool:
+        """Replay each FD phase as a captured HIP graph (swd_engine_set_graph); returns the previous setting."""
+        return bool(lib().swd_engine_set_graph(self.h, int(bool(on))))
+
     def set_fd_path(self, mode: int) -> int:
         return lib().swd_engine_set_fd_path(self.h, int(mode))
 
