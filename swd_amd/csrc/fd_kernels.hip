@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_march(
     if (it < 0 || it >= J.nt) return;
     const int rd = (launch - J.launch0) & 1, wr = rd ^ 1;  // ping-pong buffers of this launch
     const int lane = threadIdx.x & (kMarchLanes - 1);
-    const int tile = bm.tile0 + (threadIdx.x >> 6);
+    const int tile = bm.tile0 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar row loop
     if (tile >= J.tiles_x * J.tiles_z) return;
     const int tix = tile % J.tiles_x, tiz = tile / J.tiles_x;
     const int nzbc = J.nzbc, nxbc = J.nxbc;
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_march(
     if (k < 0 || it < 0) return;
     const int rd = (launch - J.launch0) & 1, wr = rd ^ 1;
     const int lane = threadIdx.x & (kMarchLanes - 1);
-    const int tile = bm.tile0 + (threadIdx.x >> 6);
+    const int tile = bm.tile0 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar row loop
     if (tile >= J.tiles_x * J.tiles_z) return;
     const int tix = tile % J.tiles_x, tiz = tile / J.tiles_x;
     const int nzbc = J.nzbc, nxbc = J.nxbc;

@@ -62,6 +62,9 @@ __device__ __forceinline__ double tb_right(double v)
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// a step index clamped into [0, nt) (a level past nt reads a valid source sample it does not use)
+__device__ __forceinline__ int itl_clamp(int itl, int nt) { return min(itl, nt - 1); }
+
 // coefficient record of one row for this lane
 template <typename T>
 struct CoefRow {
@@ -111,7 +114,9 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
     const int it0 = step - J.start;
     if (it0 < 0 || it0 >= J.nt) return;
     const int lane = threadIdx.x & (kMarchLanes - 1);
-    const int tile = bm.tile0 + (threadIdx.x >> 6);
+    // the wave index is uniform: read it into a scalar register so the row
+    // loop, its bounds and every per-row condition stay scalar
+    const int tile = bm.tile0 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (tile >= J.tiles_x * J.tiles_z) return;
     const int rd = (launch - J.launch0) & 1;
     const int tix = tile % J.tiles_x, tiz = tile / J.tiles_x;
@@ -141,29 +146,26 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
     const unsigned rowb = (unsigned)(P * sizeof(T));
     const unsigned xb = (unsigned)(xs * sizeof(T));
     const T dtx = J.dtx;
-    const int sz = J.sz, sx = J.sx, pt = J.pt, gz = J.gz;
-    const bool src5 = J.stype == 5;
+    // source column; off the grid when the job has no type-5 source
+    const int sz = J.sz, sx = J.stype == 5 ? J.sx : -(1 << 30), pt = J.pt, gz = J.gz;
     const int nt = J.nt;
     const int ng = J.ng, in_wf = J.in_wf, nsf = J.nsf, nzi = J.nz, nxi = J.nx, nbc = J.nbc;
     const gptr<T> trace_w = (gptr<T>)J.trace_w;
     const gptr<T> trace_u = (gptr<T>)J.trace_u;
     const gptr<T> snap = (gptr<T>)J.snap;
 
-    // per-level step, source value and validity
+    // per-level source value (levels past nt compute but record nothing)
     T sv[NT];
-    bool valid[NT];
 #pragma unroll
-    for (int l = 0; l < NT; ++l) {
-        const int itl = it0 + l;
-        valid[l] = itl < nt;
-        sv[l] = valid[l] ? J.src[itl] : T(0);
-    }
+    for (int l = 0; l < NT; ++l) sv[l] = it0 + l < nt ? J.src[itl_clamp(it0 + l, nt)] : T(0);
+    // receiver index and snapshot column of this lane's output column, -1
+    // where it records nothing (integers, so no lane masks stay live)
     int gi = -1;
     {
         const int d = x - J.gx;
-        if (d >= 0 && d % J.dg == 0 && d / J.dg < J.ng) gi = d / J.dg;
+        if (out && d >= 0 && d % J.dg == 0 && d / J.dg < J.ng) gi = d / J.dg;
     }
-    const bool xint = x >= J.nbc && x < J.nbc + J.nx;
+    const int sxc = (out && x >= J.nbc && x < J.nbc + J.nx) ? x - J.nbc : -1;
 
     // carried per level l (1..NT, stored at index l-1)
     T cxx[NT], czz[NT], cxz[NT], cwin[NT], cuo[NT], cwo[NT];
@@ -195,7 +197,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
             // ww_new(r)
             const T xz_l = tb_left(cxz[l]);
             T w_r = cell_vel(cr_.temp, cwin[l], cr_.b1, cxz[l] - xz_l + izz - czz[l]);
-            if (src5 && r == sz && x == sx) w_r = sv[l];
+            if (r == sz && x == sx) w_r = sv[l];
             // stress(r)
             const T u_r = cuo[l];
             const T uu_l = tb_left(u_r);
@@ -208,17 +210,18 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
             const T oxx = cell_stress2(tc, cxx[l], cr_.ca, fux, cr_.cl, bwz, dtx);
             const T ozz = (r == pt) ? T(0) : cell_stress2(tc, czz[l], cr_.ca, bwz, cr_.cl, fux, dtx);
             const T oxz = cell_shear(tc, cxz[l], cr_.cm1, fuz + bwx, dtx);
-            // records of level l+1 (step it0 + l) owned by this wavefront
-            if (out && valid[l] && r >= z0 && r < z1) {
+            // records of level l+1 (step it0 + l) owned by this wavefront;
+            // the row and step tests are wave-uniform and come first
+            if (it0 + l < nt && r >= z0 && r < z1) {
                 const int itl = it0 + l;
                 if (r == gz && gi >= 0) {
                     const size_t ti = (size_t)itl * ng + gi;
                     trace_w[ti] = w_r;
                     if (trace_u) trace_u[ti] = u_r;
                 }
-                if (nsf > 0 && (itl % in_wf) == 0 && xint && r > pt && r <= pt + nzi) {
+                if (nsf > 0 && (itl % in_wf) == 0 && r > pt && r <= pt + nzi && sxc >= 0) {
                     const size_t plane = (size_t)nzi * nxi;
-                    const size_t so = (size_t)(itl / in_wf) * nsf * plane + (size_t)(r - pt - 1) * nxi + (x - nbc);
+                    const size_t so = (size_t)(itl / in_wf) * nsf * plane + (size_t)(r - pt - 1) * nxi + sxc;
                     if (nsf == 3) {
                         snap[so] = fux;
                         snap[so + plane] = bwz;
@@ -240,7 +243,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
         }
         // final level: row i - NT
         const int rf = i - NT;
-        if (out && rf >= z0 && rf < z1) {
+        if (rf >= z0 && rf < z1 && out) {
             const unsigned o = (unsigned)rf * rowb + xb;
             gst(xx1, o, ixx);
             gst(zz1, o, izz);
@@ -275,7 +278,9 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
     const int k0 = step - J.start;
     if (k0 < 0 || J.nt - 2 - k0 < 0) return;
     const int lane = threadIdx.x & (kMarchLanes - 1);
-    const int tile = bm.tile0 + (threadIdx.x >> 6);
+    // the wave index is uniform: read it into a scalar register so the row
+    // loop, its bounds and every per-row condition stay scalar
+    const int tile = bm.tile0 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (tile >= J.tiles_x * J.tiles_z) return;
     const int rd = (launch - J.launch0) & 1;
     const int tix = tile % J.tiles_x, tiz = tile / J.tiles_x;
@@ -320,32 +325,32 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
         const int d = x - J.gx;
         if (d >= 0 && d % J.dg == 0 && d / J.dg < J.ng) gi = d / J.dg;
     }
+    // adjoint step of each level (< 0: past the end, nothing recorded)
     int itl[NT];
-    bool valid[NT];
     T inj[NT];
 #pragma unroll
     for (int l = 0; l < NT; ++l) {
         itl[l] = J.nt - 2 - (k0 + l);
-        valid[l] = itl[l] >= 0;
-        inj[l] = (valid[l] && gi >= 0) ? J.dt * J.src[(size_t)itl[l] * J.ng + gi] : T(0);
+        inj[l] = (itl[l] >= 0 && gi >= 0) ? J.dt * J.src[(size_t)itl[l] * J.ng + gi] : T(0);
     }
-    const bool xint = x >= J.nbc && x < J.nbc + J.nx;
-    const bool img_lane = out && xint;
+    // image column of this lane's output column, -1 where it images nothing
+    // (an integer, so no lane mask stays live)
+    const int icol = (out && x >= J.nbc && x < J.nbc + J.nx) ? x - J.nbc : -1;
     // Imaging levels of this launch.  With exactly one (the usual case,
     // in_wf >= NT) its snapshot and image values are prefetched one row
     // ahead with the state; otherwise every imaging level loads inline.
     int limg = -1, nimg = 0;
 #pragma unroll
     for (int l = 0; l < NT; ++l)
-        if (valid[l] && (itl[l] % in_wf) == 0) { limg = l; ++nimg; }
+        if (itl[l] >= 0 && (itl[l] % in_wf) == 0) { limg = l; ++nimg; }
     const bool pf_img = nimg == 1;
     const bool flush = pf_img && ((itl[limg] / in_wf) % kImgFlush) == 0;
     const size_t iplane = (size_t)nzi * nxi;
     const gptr<const T> snb = snap + (pf_img ? (size_t)(itl[limg] / in_wf) * nsf * iplane : 0);
     auto img_load = [&](ImgRow<T>& q, const int i) {
         const int r = i - limg - 1;
-        if (pf_img && r >= z0 && r < z1 && r > pt && r <= pt + nzi && img_lane) {
-            const size_t io = (size_t)(r - pt - 1) * nxi + (x - nbc);
+        if (pf_img && r >= z0 && r < z1 && r > pt && r <= pt + nzi && icol >= 0) {
+            const size_t io = (size_t)(r - pt - 1) * nxi + icol;
             const gptr<const T> sn = snb + io;
             q.fux = sn[0];
             if (nsf == 3) {   // [fux, bwz, fuz + bwx]
@@ -406,11 +411,11 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
             const T ozz = (r == pt) ? T(0) : cell_adj_stress(tc, czz[l], dtx, w_r - cwo[l]);
             const T oxz = cell_adj_shear(tc, cxz[l], dtx, u_q - u_r, w_r - w_l);
             if (pf_img) {
-                if (l == limg && r >= z0 && r < z1 && r > pt && r <= pt + nzi && img_lane) {
+                if (l == limg && r >= z0 && r < z1 && r > pt && r <= pt + nzi && icol >= 0) {
                     // prefetched snapshot and image values of this row
                     double c0, c1, c2;
                     image_terms(icur.fux, icur.fwz, icur.fsh, oxx, ozz, oxz, c0, c1, c2);
-                    const size_t io = (size_t)(r - pt - 1) * nxi + (x - nbc);
+                    const size_t io = (size_t)(r - pt - 1) * nxi + icol;
                     const gptr<double> im = img + io;
                     if constexpr (kPart) {
                         const gptr<T> pp = imgp + io;
@@ -439,10 +444,10 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
                         im[2 * iplane] = (double)icur.m2 + c2;
                     }
                 }
-            } else if (out && valid[l] && r >= z0 && r < z1 && (itl[l] % in_wf) == 0 && xint && r > pt &&
-                       r <= pt + nzi) {
+            } else if (itl[l] >= 0 && (itl[l] % in_wf) == 0 && r >= z0 && r < z1 && r > pt && r <= pt + nzi &&
+                       icol >= 0) {
                 const size_t plane = (size_t)nzi * nxi;
-                const size_t io = (size_t)(r - pt - 1) * nxi + (x - nbc);
+                const size_t io = (size_t)(r - pt - 1) * nxi + icol;
                 const gptr<const T> sn = snap + (size_t)(itl[l] / in_wf) * nsf * plane + io;
                 double Fux, Fwz, Fsh;
                 if (nsf == 3) {
@@ -477,7 +482,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
             ixx = oxx; izz = ozz; ixz = oxz; iu = u_r; iw = w_r;
         }
         const int rf = i - NT;
-        if (out && rf >= z0 && rf < z1) {
+        if (rf >= z0 && rf < z1 && out) {
             const unsigned o = (unsigned)rf * rowb + xb;
             gst(xx1, o, ixx);
             gst(zz1, o, izz);
