@@ -39,6 +39,13 @@ __device__ __forceinline__ T ld_row(const T* __restrict__ a, int r, size_t P, in
     return pred ? a[(size_t)r * P + xs] : T(0);
 }
 
+// Snapshots and image sums stream through non-temporal accesses, as in the
+// time-blocked kernels: they must not evict the state the next launch reads.
+template <typename T>
+__device__ __forceinline__ T stream_ld(const T* p) { return __builtin_nontemporal_load(p); }
+template <typename T>
+__device__ __forceinline__ void stream_st(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
 template <typename T>
 __device__ __forceinline__ T shfl_from_left(T v) { return __shfl_up(v, 1, kMarchLanes); }
 template <typename T>
@@ -173,14 +180,14 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_march(
                 const size_t so = (size_t)(it / J.in_wf) * J.nsf * plane +
                                   (size_t)(r - J.pt - 1) * J.nx + (x - J.nbc);
                 if (J.nsf == 3) {
-                    J.snap[so] = fux;
-                    J.snap[so + plane] = bwz;
-                    J.snap[so + 2 * plane] = fuz + bwx;
+                    stream_st(J.snap + so, fux);
+                    stream_st(J.snap + so + plane, bwz);
+                    stream_st(J.snap + so + 2 * plane, fuz + bwx);
                 } else {
-                    J.snap[so] = fux;
-                    J.snap[so + plane] = fuz;
-                    J.snap[so + 2 * plane] = bwx;
-                    J.snap[so + 3 * plane] = bwz;
+                    stream_st(J.snap + so, fux);
+                    stream_st(J.snap + so + plane, fuz);
+                    stream_st(J.snap + so + 2 * plane, bwx);
+                    stream_st(J.snap + so + 3 * plane, bwz);
                 }
             }
         }
@@ -308,9 +315,10 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_march(
                 const T* sn = J.snap + (size_t)(it / J.in_wf) * J.nsf * plane + io;
                 double Fux, Fwz, Fsh;
                 if (J.nsf == 3) {
-                    Fux = sn[0]; Fwz = sn[plane]; Fsh = sn[2 * plane];
+                    Fux = stream_ld(sn); Fwz = stream_ld(sn + plane); Fsh = stream_ld(sn + 2 * plane);
                 } else {
-                    Fux = sn[0]; Fwz = sn[3 * plane]; Fsh = (double)(sn[plane] + sn[2 * plane]);
+                    Fux = stream_ld(sn); Fwz = stream_ld(sn + 3 * plane);
+                    Fsh = (double)(stream_ld(sn + plane) + stream_ld(sn + 2 * plane));
                 }
                 double c0, c1, c2;
                 image_terms(Fux, Fwz, Fsh, xxn, zzn, xzn, c0, c1, c2);
@@ -319,19 +327,21 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_march(
                     // FP32 partials, flushed into the FP64 images every kImgFlush imaging steps
                     float* pp = J.imgp + io;
                     if (((it / J.in_wf) % kImgFlush) == 0) {
-                        im[0] = image_flush(im[0], pp[0], c0);
-                        im[plane] = image_flush(im[plane], pp[plane], c1);
-                        im[2 * plane] = image_flush(im[2 * plane], pp[2 * plane], c2);
-                        pp[0] = pp[plane] = pp[2 * plane] = 0.f;
+                        stream_st(im, image_flush(stream_ld(im), stream_ld(pp), c0));
+                        stream_st(im + plane, image_flush(stream_ld(im + plane), stream_ld(pp + plane), c1));
+                        stream_st(im + 2 * plane, image_flush(stream_ld(im + 2 * plane), stream_ld(pp + 2 * plane), c2));
+                        stream_st(pp, 0.f);
+                        stream_st(pp + plane, 0.f);
+                        stream_st(pp + 2 * plane, 0.f);
                     } else {
-                        pp[0] = image_partial(pp[0], c0, im);
-                        pp[plane] = image_partial(pp[plane], c1, im + plane);
-                        pp[2 * plane] = image_partial(pp[2 * plane], c2, im + 2 * plane);
+                        stream_st(pp, image_partial(stream_ld(pp), c0, im));
+                        stream_st(pp + plane, image_partial(stream_ld(pp + plane), c1, im + plane));
+                        stream_st(pp + 2 * plane, image_partial(stream_ld(pp + 2 * plane), c2, im + 2 * plane));
                     }
                 } else {
-                    im[0] += c0;
-                    im[plane] += c1;
-                    im[2 * plane] += c2;
+                    stream_st(im, stream_ld(im) + c0);
+                    stream_st(im + plane, stream_ld(im + plane) + c1);
+                    stream_st(im + 2 * plane, stream_ld(im + 2 * plane) + c2);
                 }
             }
         }

@@ -40,6 +40,18 @@ __device__ __forceinline__ void gst(gptr<T> base, unsigned byte_off, T v)
     *(gptr<T>)((gptr<char>)base + byte_off) = v;
 }
 
+// Streaming accesses (non-temporal): snapshots and image sums are touched
+// once per imaging step, while the next launch reads the state rows this one
+// writes.  Marking the streams non-temporal keeps them from evicting that
+// state from the L2 and the Infinity Cache (forward phase 113 -> 87 ms,
+// adjoint 157 -> 146 ms on the 60-shot production workload).  The state and
+// coefficient accesses keep the default policy: made non-temporal too, they
+// were slower.
+template <typename T>
+__device__ __forceinline__ T stream_ld(gptr<const T> p) { return __builtin_nontemporal_load(p); }
+template <typename T>
+__device__ __forceinline__ void stream_st(gptr<T> p, T v) { __builtin_nontemporal_store(v, p); }
+
 // Neighbour-lane exchange with DPP full-wave shifts (a VALU operand
 // modifier, no LDS round trip): wave_shr:1 (0x138) gives lane i the value of
 // lane i-1, wave_shl:1 (0x130) that of lane i+1.  Lanes 0 / 63 receive 0;
@@ -223,14 +235,14 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
                     const size_t plane = (size_t)nzi * nxi;
                     const size_t so = (size_t)(itl / in_wf) * nsf * plane + (size_t)(r - pt - 1) * nxi + sxc;
                     if (nsf == 3) {
-                        snap[so] = fux;
-                        snap[so + plane] = bwz;
-                        snap[so + 2 * plane] = fuz + bwx;
+                        stream_st(snap + so, fux);
+                        stream_st(snap + so + plane, bwz);
+                        stream_st(snap + so + 2 * plane, fuz + bwx);
                     } else {
-                        snap[so] = fux;
-                        snap[so + plane] = fuz;
-                        snap[so + 2 * plane] = bwx;
-                        snap[so + 3 * plane] = bwz;
+                        stream_st(snap + so, fux);
+                        stream_st(snap + so + plane, fuz);
+                        stream_st(snap + so + 2 * plane, bwx);
+                        stream_st(snap + so + 3 * plane, bwz);
                     }
                 }
             }
@@ -352,18 +364,18 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
         if (pf_img && r >= z0 && r < z1 && r > pt && r <= pt + nzi && icol >= 0) {
             const size_t io = (size_t)(r - pt - 1) * nxi + icol;
             const gptr<const T> sn = snb + io;
-            q.fux = sn[0];
+            q.fux = stream_ld(sn);
             if (nsf == 3) {   // [fux, bwz, fuz + bwx]
-                q.fwz = sn[iplane];
-                q.fsh = sn[2 * iplane];
+                q.fwz = stream_ld(sn + iplane);
+                q.fsh = stream_ld(sn + 2 * iplane);
             } else {          // [fux, fuz, bwx, bwz]
-                q.fwz = sn[3 * iplane];
-                q.fsh = sn[iplane] + sn[2 * iplane];
+                q.fwz = stream_ld(sn + 3 * iplane);
+                q.fsh = stream_ld(sn + iplane) + stream_ld(sn + 2 * iplane);
             }
             const gptr<const T> im = imgp + io;
-            q.m0 = im[0];
-            q.m1 = im[iplane];
-            q.m2 = im[2 * iplane];
+            q.m0 = stream_ld(im);
+            q.m1 = stream_ld(im + iplane);
+            q.m2 = stream_ld(im + 2 * iplane);
         }
     };
 
@@ -420,10 +432,13 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
                     if constexpr (kPart) {
                         const gptr<T> pp = imgp + io;
                         if (flush) {
-                            im[0] = image_flush(im[0], icur.m0, c0);
-                            im[iplane] = image_flush(im[iplane], icur.m1, c1);
-                            im[2 * iplane] = image_flush(im[2 * iplane], icur.m2, c2);
-                            pp[0] = pp[iplane] = pp[2 * iplane] = T(0);
+                            const gptr<const double> ic = im;
+                            stream_st(im, image_flush(stream_ld(ic), icur.m0, c0));
+                            stream_st(im + iplane, image_flush(stream_ld(ic + iplane), icur.m1, c1));
+                            stream_st(im + 2 * iplane, image_flush(stream_ld(ic + 2 * iplane), icur.m2, c2));
+                            stream_st(pp, T(0));
+                            stream_st(pp + iplane, T(0));
+                            stream_st(pp + 2 * iplane, T(0));
                         } else {
                             // image_partial for the three images, with one test on the common path
                             double v0 = (double)icur.m0 + c0, v1 = (double)icur.m1 + c1, v2 = (double)icur.m2 + c2;
@@ -434,14 +449,14 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
                             if (image_tiny(v0)) { atomicAdd(imd, v0); v0 = 0.0; }
                             if (image_tiny(v1)) { atomicAdd(imd + iplane, v1); v1 = 0.0; }
                             if (image_tiny(v2)) { atomicAdd(imd + 2 * iplane, v2); v2 = 0.0; }
-                            pp[0] = (float)v0;
-                            pp[iplane] = (float)v1;
-                            pp[2 * iplane] = (float)v2;
+                            stream_st(pp, (float)v0);
+                            stream_st(pp + iplane, (float)v1);
+                            stream_st(pp + 2 * iplane, (float)v2);
                         }
                     } else {
-                        im[0] = (double)icur.m0 + c0;
-                        im[iplane] = (double)icur.m1 + c1;
-                        im[2 * iplane] = (double)icur.m2 + c2;
+                        stream_st(im, (double)icur.m0 + c0);
+                        stream_st(im + iplane, (double)icur.m1 + c1);
+                        stream_st(im + 2 * iplane, (double)icur.m2 + c2);
                     }
                 }
             } else if (itl[l] >= 0 && (itl[l] % in_wf) == 0 && r >= z0 && r < z1 && r > pt && r <= pt + nzi &&
@@ -451,29 +466,34 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
                 const gptr<const T> sn = snap + (size_t)(itl[l] / in_wf) * nsf * plane + io;
                 double Fux, Fwz, Fsh;
                 if (nsf == 3) {
-                    Fux = sn[0]; Fwz = sn[plane]; Fsh = sn[2 * plane];
+                    Fux = stream_ld(sn); Fwz = stream_ld(sn + plane); Fsh = stream_ld(sn + 2 * plane);
                 } else {
-                    Fux = sn[0]; Fwz = sn[3 * plane]; Fsh = (double)(sn[plane] + sn[2 * plane]);
+                    Fux = stream_ld(sn); Fwz = stream_ld(sn + 3 * plane);
+                    Fsh = (double)(stream_ld(sn + plane) + stream_ld(sn + 2 * plane));
                 }
                 double c0, c1, c2;
                 image_terms(Fux, Fwz, Fsh, oxx, ozz, oxz, c0, c1, c2);
                 const gptr<double> im = img + io;
+                const gptr<const double> ic = im;
                 if constexpr (kPart) {
                     const gptr<T> pp = imgp + io;
+                    const gptr<const T> pc = pp;
                     if (((itl[l] / in_wf) % kImgFlush) == 0) {
-                        im[0] = image_flush(im[0], pp[0], c0);
-                        im[plane] = image_flush(im[plane], pp[plane], c1);
-                        im[2 * plane] = image_flush(im[2 * plane], pp[2 * plane], c2);
-                        pp[0] = pp[plane] = pp[2 * plane] = T(0);
+                        stream_st(im, image_flush(stream_ld(ic), stream_ld(pc), c0));
+                        stream_st(im + plane, image_flush(stream_ld(ic + plane), stream_ld(pc + plane), c1));
+                        stream_st(im + 2 * plane, image_flush(stream_ld(ic + 2 * plane), stream_ld(pc + 2 * plane), c2));
+                        stream_st(pp, T(0));
+                        stream_st(pp + plane, T(0));
+                        stream_st(pp + 2 * plane, T(0));
                     } else {
-                        pp[0] = image_partial(pp[0], c0, im);
-                        pp[plane] = image_partial(pp[plane], c1, im + plane);
-                        pp[2 * plane] = image_partial(pp[2 * plane], c2, im + 2 * plane);
+                        stream_st(pp, image_partial(stream_ld(pc), c0, im));
+                        stream_st(pp + plane, image_partial(stream_ld(pc + plane), c1, im + plane));
+                        stream_st(pp + 2 * plane, image_partial(stream_ld(pc + 2 * plane), c2, im + 2 * plane));
                     }
                 } else {
-                    im[0] += c0;
-                    im[plane] += c1;
-                    im[2 * plane] += c2;
+                    stream_st(im, stream_ld(ic) + c0);
+                    stream_st(im + plane, stream_ld(ic + plane) + c1);
+                    stream_st(im + 2 * plane, stream_ld(ic + 2 * plane) + c2);
                 }
             }
             cxx[l] = ixx; czz[l] = izz; cxz[l] = ixz; cwin[l] = iw;
