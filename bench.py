@@ -190,7 +190,7 @@ def iteration_leg(eng, mod, prob):
                     "snapshots, Radon, picks per trial) + model update, device-resident"}
 
 
-def field_leg():
+def field_leg(graph=True):
     """The outcrop survey at full size (fieldexamples/outcrop/outcrop.m: 40 x
     240 cells at 0.25 m, 40-cell borders, 60 shots, 60 receivers, nt 7450,
     npair 651 at df 0.1 Hz, 50-trace windows) with the outcrop variant
@@ -211,6 +211,7 @@ def field_leg():
         e.observe()
         e.set_model(mod["vp"], mod["vs"])
         tune = e.autotune()
+        e.set_graph(graph)
         e.gradient()
         t0 = time.perf_counter()
         e.gradient()
@@ -481,7 +482,7 @@ def main():
         out["dispersion"] = dispersion_leg()
         progress(f"dispersion: picks {out['dispersion']['pick_ms']} ms for 60 shots")
     if not args.no_field:
-        out["field_outcrop"] = field_leg()
+        out["field_outcrop"] = field_leg(graph=not args.no_graph)
         progress(f"outcrop survey: {out['field_outcrop']['ms_per_gradient']} ms per gradient")
     if not args.no_uneven:
         out["uneven_batch"] = uneven_leg()
