@@ -2,8 +2,9 @@
 //
 // One launch advances every shot of a batch by NT time steps.  A wavefront
 // owns 64 consecutive columns and marches down a band of rows; the NT time
-// levels run as a pipeline in registers: level l handles row i - l while
-// level 0 loads row i, so level l always sees rows r-1, r, r+1 of level l-1.
+// levels run as a pipeline in registers (the coefficient rows they read sit
+// in a per-wave LDS ring): level l handles row i - l while level 0 loads row
+// i, so level l always sees rows r-1, r, r+1 of level l-1.
 // Each level shrinks the valid lanes by one on each side (the stencil
 // radius), so a wavefront writes 64 - 2 NT columns, and a band of H output
 // rows loads H + 2 NT + 1 rows.  Intermediate levels never touch memory:
@@ -104,8 +105,8 @@ struct CoefRow {
 // grid are clamped onto the first/last row, which (like the first/last
 // column) is the zero ring of every state and coefficient plane.  B1: the
 // b1 plane differs from b and is loaded; otherwise (the wrappers set
-// b1 = b, core/staggerfd_eigen.m:106) b1 is b, one load and NT+1 ring
-// registers fewer.
+// b1 = b, core/staggerfd_eigen.m:106) b1 is b: one load and one ring plane
+// fewer.
 template <typename T, bool B1>
 struct RowLoad {
     T xx, zz, xz, uu, ww;
@@ -200,9 +201,25 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
     T cxx[NT], czz[NT], cxz[NT], cwin[NT], cuo[NT], cwo[NT];
 #pragma unroll
     for (int l = 0; l < NT; ++l) cxx[l] = czz[l] = cxz[l] = cwin[l] = cuo[l] = cwo[l] = T(0);
-    CoefRow<T> ring[NT + 1];
+    // Coefficients of rows i .. i-NT, which the NT levels read.  Float
+    // kernels keep them in a per-wave LDS ring of NT+1 rows (row k in slot
+    // k mod (NT+1); a wave only touches its own slots, and its LDS accesses
+    // stay in program order, so no barrier is needed).  That frees 5 (NT+1)
+    // VGPRs: the production forward at NT 5 and adjoint at NT 5 reach 4 waves
+    // per SIMD, and the gradient got 11 ms faster.  Double kernels keep the
+    // ring in registers (their LDS ring would exceed 64 KB per workgroup).
+    constexpr bool kLds = std::is_same<T, float>::value;
+    constexpr int NC = B1 ? 6 : 5, NS = NT + 1;
+    __shared__ T cring_s[kLds ? kMarchWaves * NS * NC * kMarchLanes : 1];
+    T* const cwave = cring_s + (kLds ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * NS * NC * kMarchLanes + lane : 0);
+    CoefRow<T> ring[kLds ? 1 : NT + 1];
+    if constexpr (kLds) {
 #pragma unroll
-    for (int k = 0; k <= NT; ++k) ring[k] = CoefRow<T>{T(0), T(0), T(0), T(0), T(0), T(0)};
+        for (int k = 0; k < NS * NC; ++k) cwave[k * kMarchLanes] = T(0);
+    } else {
+#pragma unroll
+        for (int k = 0; k <= NT; ++k) ring[k] = CoefRow<T>{T(0), T(0), T(0), T(0), T(0), T(0)};
+    }
 
     const int i0 = z0 - NT - 1, i1 = z1 - 1 + NT;
     // row loads are issued one iteration ahead so their latency overlaps the
@@ -210,16 +227,40 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
     auto row = [&](const RowLoad<T, B1>& cur, const int i) {
         // level 0: row i of the stored state and its coefficients
         T ixx = cur.xx, izz = cur.zz, ixz = cur.xz, iu = cur.uu, iw = cur.ww;
+        CoefRow<T> cqv = cur.c;
+        if constexpr (kLds) {
+            T* w = cwave + ((i + 2 * NS) % NS) * NC * kMarchLanes;
+            w[0] = cur.c.temp;
+            w[kMarchLanes] = cur.c.b;
+            if (B1) w[2 * kMarchLanes] = cur.c.b1;
+            w[(NC - 3) * kMarchLanes] = cur.c.ca;
+            w[(NC - 2) * kMarchLanes] = cur.c.cl;
+            w[(NC - 1) * kMarchLanes] = cur.c.cm1;
+        } else {
 #pragma unroll
-        for (int k = NT; k > 0; --k) ring[k] = ring[k - 1];
-        ring[0] = cur.c;
+            for (int k = NT; k > 0; --k) ring[k] = ring[k - 1];
+            ring[0] = cur.c;
+        }
 
 #pragma unroll
         for (int l = 0; l < NT; ++l) {
             // level l+1 consumes level-l row q = i - l, produces row r = q - 1
             const int r = i - l - 1;
-            const CoefRow<T>& cq = ring[l];
-            const CoefRow<T>& cr_ = ring[l + 1];
+            CoefRow<T> crv;
+            if constexpr (kLds) {
+                const T* rr = cwave + ((i - l - 1 + 2 * NS) % NS) * NC * kMarchLanes;
+                crv.temp = rr[0];
+                crv.b = rr[kMarchLanes];
+                crv.b1 = B1 ? rr[2 * kMarchLanes] : crv.b;
+                crv.ca = rr[(NC - 3) * kMarchLanes];
+                crv.cl = rr[(NC - 2) * kMarchLanes];
+                crv.cm1 = rr[(NC - 1) * kMarchLanes];
+            } else {
+                cqv = ring[l];
+                crv = ring[l + 1];
+            }
+            const CoefRow<T>& cq = cqv;
+            const CoefRow<T>& cr_ = crv;
             // uu_new(q)
             const T xx_r = tb_right(ixx);
             const T u_q = cell_vel(cq.temp, iu, cq.b, xx_r - ixx + ixz - cxz[l]);
@@ -269,6 +310,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
             cwo[l] = w_r;
             // level-(l+1) row r becomes the input of the next level
             ixx = oxx; izz = ozz; ixz = oxz; iu = u_r; iw = w_r;
+            if constexpr (kLds) cqv = crv;
         }
         // final level: row i - NT
         const int rf = i - NT;
@@ -403,22 +445,62 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
     T cxx[NT], czz[NT], cxz[NT], cwin[NT], cuo[NT], cwo[NT];
 #pragma unroll
     for (int l = 0; l < NT; ++l) cxx[l] = czz[l] = cxz[l] = cwin[l] = cuo[l] = cwo[l] = T(0);
-    CoefRow<T> ring[NT + 1];
+    // Coefficients of rows i .. i-NT, which the NT levels read.  Float
+    // kernels keep them in a per-wave LDS ring of NT+1 rows (row k in slot
+    // k mod (NT+1); a wave only touches its own slots, and its LDS accesses
+    // stay in program order, so no barrier is needed).  That frees 5 (NT+1)
+    // VGPRs: the production forward at NT 5 and adjoint at NT 5 reach 4 waves
+    // per SIMD, and the gradient got 11 ms faster.  Double kernels keep the
+    // ring in registers (their LDS ring would exceed 64 KB per workgroup).
+    constexpr bool kLds = std::is_same<T, float>::value;
+    constexpr int NC = B1 ? 6 : 5, NS = NT + 1;
+    __shared__ T cring_s[kLds ? kMarchWaves * NS * NC * kMarchLanes : 1];
+    T* const cwave = cring_s + (kLds ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * NS * NC * kMarchLanes + lane : 0);
+    CoefRow<T> ring[kLds ? 1 : NT + 1];
+    if constexpr (kLds) {
 #pragma unroll
-    for (int k = 0; k <= NT; ++k) ring[k] = CoefRow<T>{T(0), T(0), T(0), T(0), T(0), T(0)};
+        for (int k = 0; k < NS * NC; ++k) cwave[k * kMarchLanes] = T(0);
+    } else {
+#pragma unroll
+        for (int k = 0; k <= NT; ++k) ring[k] = CoefRow<T>{T(0), T(0), T(0), T(0), T(0), T(0)};
+    }
 
     const int i0 = z0 - NT - 1, i1 = z1 - 1 + NT;
     auto row = [&](const RowLoad<T, B1>& cur, const ImgRow<T>& icur, const int i) {
         T ixx = cur.xx, izz = cur.zz, ixz = cur.xz, iu = cur.uu, iw = cur.ww;
+        CoefRow<T> cqv = cur.c;
+        if constexpr (kLds) {
+            T* w = cwave + ((i + 2 * NS) % NS) * NC * kMarchLanes;
+            w[0] = cur.c.temp;
+            w[kMarchLanes] = cur.c.b;
+            if (B1) w[2 * kMarchLanes] = cur.c.b1;
+            w[(NC - 3) * kMarchLanes] = cur.c.ca;
+            w[(NC - 2) * kMarchLanes] = cur.c.cl;
+            w[(NC - 1) * kMarchLanes] = cur.c.cm1;
+        } else {
 #pragma unroll
-        for (int k = NT; k > 0; --k) ring[k] = ring[k - 1];
-        ring[0] = cur.c;
+            for (int k = NT; k > 0; --k) ring[k] = ring[k - 1];
+            ring[0] = cur.c;
+        }
 
 #pragma unroll
         for (int l = 0; l < NT; ++l) {
             const int r = i - l - 1;
-            const CoefRow<T>& cq = ring[l];
-            const CoefRow<T>& cr_ = ring[l + 1];
+            CoefRow<T> crv;
+            if constexpr (kLds) {
+                const T* rr = cwave + ((i - l - 1 + 2 * NS) % NS) * NC * kMarchLanes;
+                crv.temp = rr[0];
+                crv.b = rr[kMarchLanes];
+                crv.b1 = B1 ? rr[2 * kMarchLanes] : crv.b;
+                crv.ca = rr[(NC - 3) * kMarchLanes];
+                crv.cl = rr[(NC - 2) * kMarchLanes];
+                crv.cm1 = rr[(NC - 1) * kMarchLanes];
+            } else {
+                cqv = ring[l];
+                crv = ring[l + 1];
+            }
+            const CoefRow<T>& cq = cqv;
+            const CoefRow<T>& cr_ = crv;
             // coefficient products of input row q = r + 1
             const T caxx = cq.ca * ixx, clzz = cq.cl * izz, clxx = cq.cl * ixx, cazz = cq.ca * izz,
                     cmxz = cq.cm1 * ixz;
@@ -517,6 +599,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJo
             cuo[l] = u_q;
             cwo[l] = w_r;
             ixx = oxx; izz = ozz; ixz = oxz; iu = u_r; iw = w_r;
+            if constexpr (kLds) cqv = crv;
         }
         const int rf = i - NT;
         if (rf >= z0 && rf < z1 && out) {
