@@ -127,15 +127,15 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
     r = sub.add_parser("run")
-    r.add_argument("--fwd", default="4,33", help="forward steps per launch, band rows")
-    r.add_argument("--adj", default="4,33", help="adjoint steps per launch, band rows")
+    r.add_argument("--fwd", default="5,26", help="forward steps per launch, band rows")
+    r.add_argument("--adj", default="5,26", help="adjoint steps per launch, band rows")
     r.add_argument("--shots", type=int, default=60)
     r.add_argument("--nt", type=int, default=6000)
     r.add_argument("--gradients", type=int, default=1)
     r.add_argument("--copies", type=int, default=5)
     s = sub.add_parser("summarize")
     s.add_argument("dir")
-    s.add_argument("--shapes", default='{"fd_fwd_tb": [4, 33], "fd_adj_tb": [4, 33]}')
+    s.add_argument("--shapes", default='{"fd_fwd_tb": [5, 26], "fd_adj_tb": [5, 26]}')
     a = ap.parse_args(argv)
     (run if a.cmd == "run" else summarize)(a)
 
