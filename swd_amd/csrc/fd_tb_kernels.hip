@@ -95,6 +95,16 @@ __device__ __forceinline__ int xcd_block(int i, int n)
     return s * S + (k & 7) * G + (k >> 3);
 }
 
+// Minimum waves per SIMD the float kernels are compiled for (1 for the
+// double kernels: no constraint).  Float kernels at NT <= 5 fit 128 VGPRs
+// anyway; the hint changes the adjoint's instruction schedule (production
+// adjoint 146 -> 138 ms at 5 x 26, results bitwise equal) and brings the
+// forward at NT 6 to 4 waves (111 -> 84 ms at 6 x 26, a 52-byte spill).  At
+// NT 6 with a separate b1 plane 3 waves is what fits.  The double kernels
+// would spill hundreds of bytes under it.
+template <typename T, int NT, bool B1>
+constexpr int kTbWaves = !std::is_same<T, float>::value ? 1 : (NT >= 6 && B1) ? 3 : 4;
+
 // coefficient record of one row for this lane
 template <typename T>
 struct CoefRow {
@@ -134,7 +144,7 @@ struct ImgRow {
 };
 
 template <typename T, int NT, bool B1>
-__global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJob<T>* __restrict__ jobs,
+__global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_waves_per_eu(kTbWaves<T, NT, B1>))) void fd_fwd_tb(const FdJob<T>* __restrict__ jobs,
                                                                       const BlockMap* __restrict__ bmap, int step,
                                                                       int launch)
 {
@@ -339,7 +349,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_tb(const FdJo
 // Adjoint (reverse-time) counterpart with the imaging condition.  Level l
 // runs adjoint step it = nt - 2 - (k0 + l).
 template <typename T, int NT, bool B1>
-__global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_tb(const FdJob<T>* __restrict__ jobs,
+__global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_waves_per_eu(kTbWaves<T, NT, B1>))) void fd_adj_tb(const FdJob<T>* __restrict__ jobs,
                                                                       const BlockMap* __restrict__ bmap, int step,
                                                                       int launch)
 {
