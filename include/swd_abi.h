@@ -195,9 +195,10 @@ int swd_engine_set_fd_path(swd_engine *e, int mode);
 /* on != 0: each FD phase of the fused path replays a captured HIP graph of
  * its launch sequence (captured on first use, again after a change of
  * launch shape or batch); results are bitwise those of the eager launches.
- * swd_engine_timings then gives the dominant launch's duration as the
- * event-timed span of its graph over the launch count.  Returns the previous
- * setting. */
+ * The setting of the first engine of a swd_batch_gradient batch holds for
+ * the batch.  swd_engine_timings then gives the dominant launch's duration
+ * as the event-timed span of its graph over the launch count.  Returns the
+ * previous setting. */
 int swd_engine_set_graph(swd_engine *e, int on);
 /* Variants of the field-data scripts (fieldexamples/outcrop/outcrop.m and
  * fieldexamples/QadimahFault/QademahSWD3.m with their core/ copies); all

@@ -355,9 +355,14 @@ int run_fd_batch(const std::vector<Engine*>& engines, bool adj, bool snapshots, 
         }
         else SWD_HIP(hipMemsetAsync(e->traces.p, 0, e->traces.bytes, st));
     }
-    const Engine* e0 = engines[0];
+    Engine* e0 = engines[0];
     MarchStats ms;
-    const int rc = run_march<float>(st, jobs, adj, steps, e0->nt_block[adj], e0->tb_rows[adj], tz, &ms);
+    // the first engine's graph setting holds for the batch; it owns the
+    // captured graphs (single-engine and multi-engine batches apart)
+    const int rc = e0->use_graph
+        ? run_march_graph<float>(st, jobs, adj, steps, e0->nt_block[adj], e0->tb_rows[adj], tz,
+                                 e0->graphs[adj][snapshots][engines.size() > 1 ? 1 : 0], &ms)
+        : run_march<float>(st, jobs, adj, steps, e0->nt_block[adj], e0->tb_rows[adj], tz, &ms);
     if (rc == kOk)
         for (Engine* e : engines) {
             if (adj) { e->times.adj_spl = ms.nt_block; e->times.adj_rows = ms.rows; }
@@ -369,26 +374,6 @@ int run_fd_batch(const std::vector<Engine*>& engines, bool adj, bool snapshots, 
 
 int Engine::run_fd(bool adj, bool snapshots)
 {
-    if (fd_path == 0 && use_graph) {
-        // zero the state and outputs on the stream, then replay (or capture) the launches
-        std::vector<FdJob<float>> jobs;
-        make_jobs(jobs, adj, snapshots);
-        SWD_HIP(hipMemsetAsync(state.p, 0, state.bytes, st));
-        if (adj) {
-            SWD_HIP(hipMemsetAsync(imgs.p, 0, imgs.bytes, st));
-            SWD_HIP(hipMemsetAsync(imgp.p, 0, imgp.bytes, st));
-        } else {
-            SWD_HIP(hipMemsetAsync(traces.p, 0, traces.bytes, st));
-        }
-        MarchStats ms;
-        const int rc = run_march_graph<float>(st, jobs, adj, adj ? p.nt - 1 : p.nt, nt_block[adj], tb_rows[adj], tz,
-                                              graphs[adj][snapshots], &ms);
-        if (rc == kOk) {
-            if (adj) { times.adj_kernel_us = ms.kernel_us; times.adj_spl = ms.nt_block; times.adj_rows = ms.rows; }
-            else { times.fwd_kernel_us = ms.kernel_us; times.fwd_spl = ms.nt_block; times.fwd_rows = ms.rows; }
-        }
-        return rc;
-    }
     if (fd_path == 0) {
         MarchStats ms;
         const int rc = run_fd_batch({this}, adj, snapshots, st, tz, &ms);

@@ -46,7 +46,7 @@ public:
     double vp_ratio = 0;   // > 0: device model updates set vp = vp_ratio * vs (field scripts); 0: vp fixed (SWD.m)
     bool b1_same = false;  // the b1 plane equals b (jobs then read b for b1)
     bool use_graph = false;    // FD phases replay captured HIP graphs
-    MarchGraph graphs[2][2];   // [adjoint][snapshots]
+    MarchGraph graphs[2][2][2];   // [adjoint][snapshots][multi-engine batch]
 
     DevBuf coefs;     // 6 planes float: temp, b, b1, ca, cl, cm1
     DevBuf vs_d;      // nz x nx double (MATLAB layout) for the assembly
