@@ -24,7 +24,7 @@ def smooth2a(A, nr: int, nc: int) -> np.ndarray:
 
 def invert(problem: dict, models: dict, iterations: int, pur: float = -0.05, err: float = 0.01,
            smooth_z: int = 1, smooth_x: int = 2, min_offset: int = 3, log=None, *, offset_step: int = 3,
-           variant="swd", observed=None, vs_bounds=None) -> dict:
+           variant="swd", observed=None, vs_bounds=None, autotune: bool = False, graph: bool = True) -> dict:
     """Run SWD iterations from models["vs"] (with models["vp"]).
 
     Observed picks: by default from a forward of models["vs_true"] (synthetic
@@ -36,8 +36,11 @@ def invert(problem: dict, models: dict, iterations: int, pur: float = -0.05, err
     ``offset_step`` (SWD.m:326-329: 3; outcrop.m:342-345: 5) and the loop
     stops once it reaches ``min_offset`` or less (SWD.m:331-333; the outcrop
     script's WD gradient below offset 11 is not part of this engine).
-    ``variant`` is a field-script variant of :mod:`swd_amd.field`.  Returns
-    the final model, the residual history and the per-iteration records."""
+    ``variant`` is a field-script variant of :mod:`swd_amd.field`.
+    ``autotune`` times the FD launch shapes for each engine (offset window)
+    before its first iteration; ``graph`` replays the FD phases as captured
+    HIP graphs.  Neither changes the results.  Returns the final model, the
+    residual history and the per-iteration records."""
     if vs_bounds is None:
         vs_bounds = (float(np.min(models["vs_true"])), float(np.max(models["vs_true"])))
     vsmin, vsmax = (float(v) for v in vs_bounds)
@@ -57,6 +60,9 @@ def invert(problem: dict, models: dict, iterations: int, pur: float = -0.05, err
         else:
             e.set_observed_picks(*observed(q["offset"], e.npair))
         e.set_model(vp_now, vs_now)
+        if autotune:
+            e.autotune()
+        e.set_graph(graph)
         return e
 
     e = engine_for(prob, vp, vs)
