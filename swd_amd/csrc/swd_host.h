@@ -230,7 +230,8 @@ int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adjoint, int ste
 // count (so it includes the gaps between its kernels).
 struct MarchGraph {
     std::vector<unsigned char> key;  // launch parameters and job tables it was captured for
-    DevBuf djt, dbt, djs, dbs;       // the job tables and block maps its kernels read
+    std::vector<DevBuf> djt, dbt;    // job tables and block maps of the time-blocked segments
+    DevBuf djs, dbs;                 // those of the single-step remainder
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};

@@ -71,6 +71,47 @@ int auto_band_rows(const std::vector<FdJob<T>>& jobs, bool adj, int NT)
 template int auto_band_rows<float>(const std::vector<FdJob<float>>&, bool, int);
 template int auto_band_rows<double>(const std::vector<FdJob<double>>&, bool, int);
 
+// Time-blocked launches 0 .. nL-1 of a batch, in segments of constant job
+// set: a job takes part in the launches up to its last step (ceil(steps_j /
+// NT), steps_j = nt forward, nt - 1 adjoint) and leaves the block map after
+// them.  The first segment (every job) uses tb_rows (from occupancy when <=
+// 0); a later segment, with fewer jobs, picks its band height from
+// occupancy, so the tail of an uneven batch still fills the GPU.  Tiling
+// does not change results: every band height gives the same bits.
+template <typename T>
+struct TbSegment {
+    int L0 = 0, L1 = 0, rows = 0;
+    std::vector<FdJob<T>> jobs;
+    MarchPlan plan;
+};
+
+template <typename T>
+static int tb_segments(const std::vector<FdJob<T>>& jobs, bool adj, int NT, int nL, int tb_rows,
+                       std::vector<TbSegment<T>>& segs)
+{
+    segs.clear();
+    std::vector<int> last(jobs.size());
+    for (size_t j = 0; j < jobs.size(); ++j)
+        last[j] = std::min(nL, div_up(std::max(adj ? jobs[j].nt - 1 : jobs[j].nt, 0), NT));
+    std::vector<int> cuts = last;
+    std::sort(cuts.begin(), cuts.end());
+    cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+    int L0 = 0, rc;
+    for (int c : cuts) {
+        if (c <= L0) continue;
+        TbSegment<T> sg;
+        sg.L0 = L0;
+        sg.L1 = c;
+        for (size_t j = 0; j < jobs.size(); ++j)
+            if (last[j] > L0) sg.jobs.push_back(jobs[j]);
+        sg.rows = (L0 == 0 && tb_rows > 0) ? tb_rows : auto_band_rows(sg.jobs, adj, NT);
+        if ((rc = build_march_plan(sg.jobs, sg.rows, sg.plan, kMarchLanes - 2 * NT))) return rc;
+        segs.push_back(std::move(sg));
+        L0 = c;
+    }
+    return kOk;
+}
+
 template <typename T>
 int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int steps, int nt_block, int tb_rows,
               int ss_rows, MarchStats* stats)
@@ -86,16 +127,20 @@ int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int steps, 
     const int nL = NT > 1 ? steps / NT : 0;
     const int rem = steps - nL * NT;
     int rc;
-    std::vector<FdJob<T>> jt = jobs, js = jobs;
-    MarchPlan ptb, pss;
-    DevBuf djt, dbt, djs, dbs;
-    if (nL > 0 && tb_rows <= 0) tb_rows = auto_band_rows(jobs, adj, NT);
+    std::vector<TbSegment<T>> segs;
+    std::vector<DevBuf> djt, dbt;
     if (nL > 0) {
-        if ((rc = build_march_plan(jt, tb_rows, ptb, kMarchLanes - 2 * NT))) return rc;
-        if ((rc = djt.upload(jt.data(), jt.size() * sizeof(FdJob<T>), st)) ||
-            (rc = dbt.upload(ptb.bmap.data(), ptb.bmap.size() * sizeof(BlockMap), st)))
-            return rc;
+        if ((rc = tb_segments(jobs, adj, NT, nL, tb_rows, segs))) return rc;
+        djt.resize(segs.size());
+        dbt.resize(segs.size());
+        for (size_t k = 0; k < segs.size(); ++k)
+            if ((rc = djt[k].upload(segs[k].jobs.data(), segs[k].jobs.size() * sizeof(FdJob<T>), st)) ||
+                (rc = dbt[k].upload(segs[k].plan.bmap.data(), segs[k].plan.bmap.size() * sizeof(BlockMap), st)))
+                return rc;
     }
+    std::vector<FdJob<T>> js = jobs;
+    MarchPlan pss;
+    DevBuf djs, dbs;
     if (rem > 0) {
         if ((rc = build_march_plan(js, ss_rows, pss, kMarchOut))) return rc;
         if ((rc = djs.upload(js.data(), js.size() * sizeof(FdJob<T>), st)) ||
@@ -112,13 +157,19 @@ int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int steps, 
     rc = kOk;
     int launch = 0;
     SWD_HIP(hipEventRecord(ev[2 * nsamp], st));
-    for (int L = 0; L < nL && rc == kOk; ++L, ++launch) {
-        const bool samp = L % stride == 0;
-        if (samp) SWD_HIP(hipEventRecord(ev[2 * (L / stride)], st));
-        rc = adj ? launch_adj_tb<T>(st, djt.as<FdJob<T>>(), dbt.as<BlockMap>(), ptb.nblocks, L * NT, launch, NT, b1)
-                 : launch_fwd_tb<T>(st, djt.as<FdJob<T>>(), dbt.as<BlockMap>(), ptb.nblocks, L * NT, launch, NT, b1);
-        if (samp && rc == kOk) SWD_HIP(hipEventRecord(ev[2 * (L / stride) + 1], st));
+    for (size_t k = 0; k < segs.size() && rc == kOk; ++k) {
+        const TbSegment<T>& sg = segs[k];
+        for (int L = sg.L0; L < sg.L1 && rc == kOk; ++L, ++launch) {
+            const bool samp = L % stride == 0;
+            if (samp) SWD_HIP(hipEventRecord(ev[2 * (L / stride)], st));
+            rc = adj ? launch_adj_tb<T>(st, djt[k].as<FdJob<T>>(), dbt[k].as<BlockMap>(), sg.plan.nblocks, L * NT,
+                                        launch, NT, b1)
+                     : launch_fwd_tb<T>(st, djt[k].as<FdJob<T>>(), dbt[k].as<BlockMap>(), sg.plan.nblocks, L * NT,
+                                        launch, NT, b1);
+            if (samp && rc == kOk) SWD_HIP(hipEventRecord(ev[2 * (L / stride) + 1], st));
+        }
     }
+    launch = nL;
     for (int j = 0; j < rem && rc == kOk; ++j, ++launch) {
         const bool samp = nL == 0 && j % stride == 0;
         if (samp) SWD_HIP(hipEventRecord(ev[2 * (j / stride)], st));
@@ -144,7 +195,7 @@ int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int steps, 
         stats->kernel_us = nsamp > 0 ? 1000.f * tot / nsamp : 0.f;
         stats->span_ms = span;
         stats->nt_block = nL > 0 ? NT : 1;
-        stats->rows = nL > 0 ? tb_rows : ss_rows;
+        stats->rows = nL > 0 ? segs[0].rows : ss_rows;
     }
     return kOk;
 }
@@ -172,26 +223,29 @@ int run_march_graph(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int s
     const int nL = NT > 1 ? steps / NT : 0;
     const int rem = steps - nL * NT;
     int rc;
-    if (nL > 0 && tb_rows <= 0) tb_rows = auto_band_rows(jobs, adj, NT);
-    std::vector<FdJob<T>> jt = jobs, js = jobs;
-    MarchPlan ptb, pss;
-    if (nL > 0 && (rc = build_march_plan(jt, tb_rows, ptb, kMarchLanes - 2 * NT))) return rc;
-    if (rem > 0 && (rc = build_march_plan(js, ss_rows, pss, kMarchOut))) return rc;
+    // the key: launch parameters and the jobs as given (the segments and
+    // plans are a function of them)
     std::vector<unsigned char> key;
     key_add(key, adj); key_add(key, steps); key_add(key, NT); key_add(key, tb_rows); key_add(key, ss_rows);
     key_add(key, b1);
-    const unsigned char* pj = (const unsigned char*)jt.data();
-    key.insert(key.end(), pj, pj + jt.size() * sizeof(FdJob<T>));
-    pj = (const unsigned char*)js.data();
-    key.insert(key.end(), pj, pj + js.size() * sizeof(FdJob<T>));
+    const unsigned char* pj = (const unsigned char*)jobs.data();
+    key.insert(key.end(), pj, pj + jobs.size() * sizeof(FdJob<T>));
     for (auto& e : g.ev)
         if (!e) SWD_HIP(hipEventCreate(&e));
     if (!g.ready || key != g.key) {
         g.reset();
+        std::vector<TbSegment<T>> segs;
+        if (nL > 0 && (rc = tb_segments(jobs, adj, NT, nL, tb_rows, segs))) return rc;
+        std::vector<FdJob<T>> js = jobs;
+        MarchPlan pss;
+        if (rem > 0 && (rc = build_march_plan(js, ss_rows, pss, kMarchOut))) return rc;
         // the tables go up before the capture: the graphs' kernels read them at every replay
-        if (nL > 0 && ((rc = g.djt.upload(jt.data(), jt.size() * sizeof(FdJob<T>), st)) ||
-                       (rc = g.dbt.upload(ptb.bmap.data(), ptb.bmap.size() * sizeof(BlockMap), st))))
-            return rc;
+        g.djt.resize(segs.size());
+        g.dbt.resize(segs.size());
+        for (size_t k = 0; k < segs.size(); ++k)
+            if ((rc = g.djt[k].upload(segs[k].jobs.data(), segs[k].jobs.size() * sizeof(FdJob<T>), st)) ||
+                (rc = g.dbt[k].upload(segs[k].plan.bmap.data(), segs[k].plan.bmap.size() * sizeof(BlockMap), st)))
+                return rc;
         if (rem > 0 && ((rc = g.djs.upload(js.data(), js.size() * sizeof(FdJob<T>), st)) ||
                         (rc = g.dbs.upload(pss.bmap.data(), pss.bmap.size() * sizeof(BlockMap), st))))
             return rc;
@@ -202,13 +256,19 @@ int run_march_graph(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int s
             if (n == 0) continue;
             SWD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             rc = kOk;
-            for (int i = 0; i < n && rc == kOk; ++i) {
-                if (part == 0)
-                    rc = adj ? launch_adj_tb<T>(st, g.djt.as<FdJob<T>>(), g.dbt.as<BlockMap>(), ptb.nblocks, i * NT, i, NT, b1)
-                             : launch_fwd_tb<T>(st, g.djt.as<FdJob<T>>(), g.dbt.as<BlockMap>(), ptb.nblocks, i * NT, i, NT, b1);
-                else
-                    rc = adj ? launch_adj_march<T>(st, g.djs.as<FdJob<T>>(), g.dbs.as<BlockMap>(), pss.nblocks, nL * NT + i, nL + i)
-                             : launch_fwd_march<T>(st, g.djs.as<FdJob<T>>(), g.dbs.as<BlockMap>(), pss.nblocks, nL * NT + i, nL + i);
+            if (part == 0) {
+                for (size_t k = 0; k < segs.size() && rc == kOk; ++k)
+                    for (int L = segs[k].L0; L < segs[k].L1 && rc == kOk; ++L)
+                        rc = adj ? launch_adj_tb<T>(st, g.djt[k].as<FdJob<T>>(), g.dbt[k].as<BlockMap>(),
+                                                    segs[k].plan.nblocks, L * NT, L, NT, b1)
+                                 : launch_fwd_tb<T>(st, g.djt[k].as<FdJob<T>>(), g.dbt[k].as<BlockMap>(),
+                                                    segs[k].plan.nblocks, L * NT, L, NT, b1);
+            } else {
+                for (int i = 0; i < n && rc == kOk; ++i)
+                    rc = adj ? launch_adj_march<T>(st, g.djs.as<FdJob<T>>(), g.dbs.as<BlockMap>(), pss.nblocks,
+                                                   nL * NT + i, nL + i)
+                             : launch_fwd_march<T>(st, g.djs.as<FdJob<T>>(), g.dbs.as<BlockMap>(), pss.nblocks,
+                                                   nL * NT + i, nL + i);
             }
             hipGraph_t graph = nullptr;
             const hipError_t ce = hipStreamEndCapture(st, &graph);
@@ -225,7 +285,7 @@ int run_march_graph(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int s
         g.key.swap(key);
         g.ready = true;
         g.nt_block = nL > 0 ? NT : 1;
-        g.rows = nL > 0 ? tb_rows : ss_rows;
+        g.rows = nL > 0 ? segs[0].rows : ss_rows;
     }
     SWD_HIP(hipEventRecord(g.ev[0], st));
     if (g.exec[0]) SWD_HIP(hipGraphLaunch(g.exec[0], st));
