@@ -325,18 +325,18 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_march(
                 double* im = J.img + io;
                 if constexpr (std::is_same<T, float>::value) {
                     // FP32 partials, flushed into the FP64 images every kImgFlush imaging steps
-                    float* pp = J.imgp + io;
+                    float* pp = J.imgp + 3 * io;
                     if (((it / J.in_wf) % kImgFlush) == 0) {
                         stream_st(im, image_flush(stream_ld(im), stream_ld(pp), c0));
-                        stream_st(im + plane, image_flush(stream_ld(im + plane), stream_ld(pp + plane), c1));
-                        stream_st(im + 2 * plane, image_flush(stream_ld(im + 2 * plane), stream_ld(pp + 2 * plane), c2));
+                        stream_st(im + plane, image_flush(stream_ld(im + plane), stream_ld(pp + 1), c1));
+                        stream_st(im + 2 * plane, image_flush(stream_ld(im + 2 * plane), stream_ld(pp + 2), c2));
                         stream_st(pp, 0.f);
-                        stream_st(pp + plane, 0.f);
-                        stream_st(pp + 2 * plane, 0.f);
+                        stream_st(pp + 1, 0.f);
+                        stream_st(pp + 2, 0.f);
                     } else {
                         stream_st(pp, image_partial(stream_ld(pp), c0, im));
-                        stream_st(pp + plane, image_partial(stream_ld(pp + plane), c1, im + plane));
-                        stream_st(pp + 2 * plane, image_partial(stream_ld(pp + 2 * plane), c2, im + 2 * plane));
+                        stream_st(pp + 1, image_partial(stream_ld(pp + 1), c1, im + plane));
+                        stream_st(pp + 2, image_partial(stream_ld(pp + 2), c2, im + 2 * plane));
                     }
                 } else {
                     stream_st(im, stream_ld(im) + c0);

@@ -441,10 +441,15 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
                 q.fwz = stream_ld(sn + 3 * iplane);
                 q.fsh = stream_ld(sn + iplane) + stream_ld(sn + 2 * iplane);
             }
-            const gptr<const T> im = imgp + io;
+            // FP32 partials are interleaved [cell][3]: one stream instead of
+            // three planes (production adjoint 139 -> 133 ms mean over seven
+            // processes, without the slow page-placement mode); the double
+            // kernels read their FP64 images as planes
+            const size_t ps = kPart ? 1 : iplane;
+            const gptr<const T> im = imgp + (kPart ? 3 * io : io);
             q.m0 = stream_ld(im);
-            q.m1 = stream_ld(im + iplane);
-            q.m2 = stream_ld(im + 2 * iplane);
+            q.m1 = stream_ld(im + ps);
+            q.m2 = stream_ld(im + 2 * ps);
         }
     };
 
@@ -539,15 +544,15 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
                     const size_t io = (size_t)(r - pt - 1) * nxi + icol;
                     const gptr<double> im = img + io;
                     if constexpr (kPart) {
-                        const gptr<T> pp = imgp + io;
+                        const gptr<T> pp = imgp + 3 * io;
                         if (flush) {
                             const gptr<const double> ic = im;
                             stream_st(im, image_flush(stream_ld(ic), icur.m0, c0));
                             stream_st(im + iplane, image_flush(stream_ld(ic + iplane), icur.m1, c1));
                             stream_st(im + 2 * iplane, image_flush(stream_ld(ic + 2 * iplane), icur.m2, c2));
                             stream_st(pp, T(0));
-                            stream_st(pp + iplane, T(0));
-                            stream_st(pp + 2 * iplane, T(0));
+                            stream_st(pp + 1, T(0));
+                            stream_st(pp + 2, T(0));
                         } else {
                             // image_partial for the three images, with one test on the common path
                             double v0 = (double)icur.m0 + c0, v1 = (double)icur.m1 + c1, v2 = (double)icur.m2 + c2;
@@ -559,8 +564,8 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
                             if (image_tiny(v1)) { atomicAdd(imd + iplane, v1); v1 = 0.0; }
                             if (image_tiny(v2)) { atomicAdd(imd + 2 * iplane, v2); v2 = 0.0; }
                             stream_st(pp, (float)v0);
-                            stream_st(pp + iplane, (float)v1);
-                            stream_st(pp + 2 * iplane, (float)v2);
+                            stream_st(pp + 1, (float)v1);
+                            stream_st(pp + 2, (float)v2);
                         }
                     } else {
                         stream_st(im, (double)icur.m0 + c0);
@@ -585,19 +590,19 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
                 const gptr<double> im = img + io;
                 const gptr<const double> ic = im;
                 if constexpr (kPart) {
-                    const gptr<T> pp = imgp + io;
+                    const gptr<T> pp = imgp + 3 * io;
                     const gptr<const T> pc = pp;
                     if (((itl[l] / in_wf) % kImgFlush) == 0) {
                         stream_st(im, image_flush(stream_ld(ic), stream_ld(pc), c0));
-                        stream_st(im + plane, image_flush(stream_ld(ic + plane), stream_ld(pc + plane), c1));
-                        stream_st(im + 2 * plane, image_flush(stream_ld(ic + 2 * plane), stream_ld(pc + 2 * plane), c2));
+                        stream_st(im + plane, image_flush(stream_ld(ic + plane), stream_ld(pc + 1), c1));
+                        stream_st(im + 2 * plane, image_flush(stream_ld(ic + 2 * plane), stream_ld(pc + 2), c2));
                         stream_st(pp, T(0));
-                        stream_st(pp + plane, T(0));
-                        stream_st(pp + 2 * plane, T(0));
+                        stream_st(pp + 1, T(0));
+                        stream_st(pp + 2, T(0));
                     } else {
                         stream_st(pp, image_partial(stream_ld(pc), c0, im));
-                        stream_st(pp + plane, image_partial(stream_ld(pc + plane), c1, im + plane));
-                        stream_st(pp + 2 * plane, image_partial(stream_ld(pc + 2 * plane), c2, im + 2 * plane));
+                        stream_st(pp + 1, image_partial(stream_ld(pc + 1), c1, im + plane));
+                        stream_st(pp + 2, image_partial(stream_ld(pc + 2), c2, im + 2 * plane));
                     }
                 } else {
                     stream_st(im, stream_ld(ic) + c0);

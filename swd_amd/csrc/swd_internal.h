@@ -63,7 +63,7 @@ struct FdJob {
     T* snap;
     // adjoint images [3][nz][nx] (cl_img, cm_img, illum), accumulated in double
     double* img;
-    // FP32 partial sums of the images, [3][nz][nx] (required by the fused
+    // FP32 partial sums of the images, [nz][nx][3] (required by the fused
     // float kernels, unused by the others): every imaging step adds into
     // them, and imaging step k with k % kImgFlush == 0 (the last one, it = 0,
     // included) moves them into img and clears them
