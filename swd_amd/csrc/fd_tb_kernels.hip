@@ -301,11 +301,13 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
                 }
                 if (nsf > 0 && (itl % in_wf) == 0 && r > pt && r <= pt + nzi && sxc >= 0) {
                     const size_t plane = (size_t)nzi * nxi;
-                    const size_t so = (size_t)(itl / in_wf) * nsf * plane + (size_t)(r - pt - 1) * nxi + sxc;
-                    if (nsf == 3) {
-                        stream_st(snap + so, fux);
-                        stream_st(snap + so + plane, bwz);
-                        stream_st(snap + so + 2 * plane, fuz + bwx);
+                    const size_t sb = (size_t)(itl / in_wf) * nsf * plane;
+                    const size_t so = sb + (size_t)(r - pt - 1) * nxi + sxc;
+                    if (nsf == 3) {   // [row][fux, bwz, fuz + bwx][x]
+                        const size_t sr = sb + (size_t)(r - pt - 1) * 3 * nxi + sxc;
+                        stream_st(snap + sr, fux);
+                        stream_st(snap + sr + nxi, bwz);
+                        stream_st(snap + sr + 2 * nxi, fuz + bwx);
                     } else {
                         stream_st(snap + so, fux);
                         stream_st(snap + so + plane, fuz);
@@ -432,12 +434,14 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
         const int r = i - limg - 1;
         if (pf_img && r >= z0 && r < z1 && r > pt && r <= pt + nzi && icol >= 0) {
             const size_t io = (size_t)(r - pt - 1) * nxi + icol;
-            const gptr<const T> sn = snb + io;
-            q.fux = stream_ld(sn);
-            if (nsf == 3) {   // [fux, bwz, fuz + bwx]
-                q.fwz = stream_ld(sn + iplane);
-                q.fsh = stream_ld(sn + 2 * iplane);
-            } else {          // [fux, fuz, bwx, bwz]
+            if (nsf == 3) {   // [row][fux, bwz, fuz + bwx][x]
+                const gptr<const T> sn = snb + (size_t)(r - pt - 1) * 3 * nxi + icol;
+                q.fux = stream_ld(sn);
+                q.fwz = stream_ld(sn + nxi);
+                q.fsh = stream_ld(sn + 2 * nxi);
+            } else {          // [fux, fuz, bwx, bwz] planes
+                const gptr<const T> sn = snb + io;
+                q.fux = stream_ld(sn);
                 q.fwz = stream_ld(sn + 3 * iplane);
                 q.fsh = stream_ld(sn + iplane) + stream_ld(sn + 2 * iplane);
             }
@@ -577,10 +581,12 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
                        icol >= 0) {
                 const size_t plane = (size_t)nzi * nxi;
                 const size_t io = (size_t)(r - pt - 1) * nxi + icol;
-                const gptr<const T> sn = snap + (size_t)(itl[l] / in_wf) * nsf * plane + io;
+                const gptr<const T> sb = snap + (size_t)(itl[l] / in_wf) * nsf * plane;
+                const gptr<const T> sn = sb + io;
                 double Fux, Fwz, Fsh;
                 if (nsf == 3) {
-                    Fux = stream_ld(sn); Fwz = stream_ld(sn + plane); Fsh = stream_ld(sn + 2 * plane);
+                    const gptr<const T> sc = sb + (size_t)(r - pt - 1) * 3 * nxi + icol;
+                    Fux = stream_ld(sc); Fwz = stream_ld(sc + nxi); Fsh = stream_ld(sc + 2 * nxi);
                 } else {
                     Fux = stream_ld(sn); Fwz = stream_ld(sn + 3 * plane);
                     Fsh = (double)(stream_ld(sn + plane) + stream_ld(sn + 2 * plane));

@@ -54,10 +54,10 @@ public:
     DevBuf state;     // nshots x 10 planes float
     DevBuf strain;    // 4 planes (plain path scratch)
     DevBuf traces;    // [shot][t][g] float
-    DevBuf snaps;     // [shot][nrec][3][nz][nx] float
+    DevBuf snaps;     // [shot][nrec][nz][3][nx] float (fux, bwz, fuz + bwx of each row)
     DevBuf adjsrc;    // [shot][t][g] float
     DevBuf imgs;      // [shot][3][nz][nx] double
-    DevBuf imgp;      // [shot][3][nz][nx] float, partial image sums of the fused kernels
+    DevBuf imgp;      // [shot][nz][nx][3] float, partial image sums of the fused kernels
     DevBuf outs;      // dk_vs, g_cl, g_cm, g_il (nz x nx double each)
     DevBuf djobs, dbmap;
     DevBuf vp_d;      // nz x nx double, the current vp (column-major)

@@ -59,7 +59,9 @@ struct FdJob {
     // forward receivers [nt][ng] (g fastest); trace_u may be null
     T* trace_w;
     T* trace_u;
-    // snapshots [nt/in_wf][nsf][nz][nx]; nsf = 3 (fux, bwz, fuz+bwx) or 4 (fux, fuz, bwx, bwz)
+    // snapshots: nsf = 3, [nt/in_wf][nz][3][nx] (fux, bwz, fuz+bwx, the three
+    // fields of a row back to back); nsf = 4, [nt/in_wf][4][nz][nx] planes
+    // (fux, fuz, bwx, bwz)
     T* snap;
     // adjoint images [3][nz][nx] (cl_img, cm_img, illum), accumulated in double
     double* img;
