@@ -34,15 +34,15 @@ def fd_bytes_per_launch(prob, nshots, adjoint):
     read) plus, averaged over the in_wf-step snapshot cycle, 16 B per interior
     cell per snapshot step (forward: strain written; adjoint: strain read) and,
     for the adjoint, the image update: an FP32 read-modify-write of the 3
-    partial images (24 B) plus, every 16th imaging step, the FP64
-    read-modify-write of the 3 images (48 B / 16)."""
+    partial images (24 B) plus, every 32nd imaging step, the FP64
+    read-modify-write of the 3 images (48 B / 32)."""
     nzbc = prob["nz"] + 2 + prob["nbc"]
     nxbc = prob["nx"] + 2 * prob["nbc"]
     cells = nzbc * nxbc
     inner = prob["nz"] * prob["nx"]
     per = 60.0 * cells + 16.0 * inner / prob["in_wf"]
     if adjoint:
-        per += (24.0 + 48.0 / 16) * inner / prob["in_wf"]
+        per += (24.0 + 48.0 / 32) * inner / prob["in_wf"]
     return per * nshots
 
 

@@ -82,7 +82,9 @@ struct FdJob {
 };
 
 // Imaging steps per FP64 flush of the FP32 partial images (FdJob::imgp).
-constexpr int kImgFlush = 16;
+// 32: dk_vs moves 3.6e-8 relative L2 against 16, and the production adjoint
+// is about 3% faster (half the flush launches).
+constexpr int kImgFlush = 32;
 
 // Block -> (job, first tile) map entry for fused launches.
 struct BlockMap {
