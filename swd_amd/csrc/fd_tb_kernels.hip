@@ -134,6 +134,69 @@ struct RowLoad {
     }
 };
 
+// Coefficients of rows i .. i-NT, which the NT levels of row i read (level l
+// writes row i - l - 1 and reads the row above it).  Float kernels keep them
+// in a per-wave LDS ring of NT+1 rows, row k in slot k mod (NT+1): a wave
+// only touches its own slots and its LDS accesses stay in program order, so
+// no barrier is needed.  That frees 5 (NT+1) VGPRs; the production forward
+// and adjoint at NT 5 reach 4 waves per SIMD, and the gradient got 11 ms
+// faster.  Double kernels keep the rows in registers (their LDS ring would
+// exceed 64 KB per workgroup).
+template <typename T, int NT, bool B1>
+struct CoefRing {
+    static constexpr bool kLds = std::is_same<T, float>::value;
+    static constexpr int NC = B1 ? 6 : 5, NS = NT + 1;
+    static constexpr int kLdsElems = kLds ? kMarchWaves * NS * NC * kMarchLanes : 1;
+    T* w = nullptr;                     // LDS: this wave's slots, at this lane
+    CoefRow<T> reg[kLds ? 1 : NT + 1];  // registers: reg[k] holds row i - k
+
+    __device__ __forceinline__ void init(T* lds, int wave, int lane)
+    {
+        if constexpr (kLds) {
+            w = lds + wave * NS * NC * kMarchLanes + lane;
+#pragma unroll
+            for (int k = 0; k < NS * NC; ++k) w[k * kMarchLanes] = T(0);
+        } else {
+#pragma unroll
+            for (int k = 0; k <= NT; ++k) reg[k] = CoefRow<T>{T(0), T(0), T(0), T(0), T(0), T(0)};
+        }
+    }
+    // row i enters (its slot last held row i - NT - 1, which no level reads any more)
+    __device__ __forceinline__ void push(int i, const CoefRow<T>& c)
+    {
+        if constexpr (kLds) {
+            T* s = w + ((i + 2 * NS) % NS) * NC * kMarchLanes;
+            s[0] = c.temp;
+            s[kMarchLanes] = c.b;
+            if (B1) s[2 * kMarchLanes] = c.b1;
+            s[(NC - 3) * kMarchLanes] = c.ca;
+            s[(NC - 2) * kMarchLanes] = c.cl;
+            s[(NC - 1) * kMarchLanes] = c.cm1;
+        } else {
+#pragma unroll
+            for (int k = NT; k > 0; --k) reg[k] = reg[k - 1];
+            reg[0] = c;
+        }
+    }
+    // coefficients of row i - l - 1, the row level l writes (l = 0 .. NT-1)
+    __device__ __forceinline__ CoefRow<T> below(int i, int l) const
+    {
+        if constexpr (kLds) {
+            const T* s = w + ((i - l - 1 + 2 * NS) % NS) * NC * kMarchLanes;
+            CoefRow<T> c;
+            c.temp = s[0];
+            c.b = s[kMarchLanes];
+            c.b1 = B1 ? s[2 * kMarchLanes] : c.b;
+            c.ca = s[(NC - 3) * kMarchLanes];
+            c.cl = s[(NC - 2) * kMarchLanes];
+            c.cm1 = s[(NC - 1) * kMarchLanes];
+            return c;
+        } else {
+            return reg[l + 1];
+        }
+    }
+};
+
 // Snapshot and image values of one imaging row for this lane, loaded one
 // row ahead like the state: raw snapshot fields and the current image sums
 // (float kernels: the FP32 partials; double kernels: the FP64 images).
@@ -211,25 +274,9 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
     T cxx[NT], czz[NT], cxz[NT], cwin[NT], cuo[NT], cwo[NT];
 #pragma unroll
     for (int l = 0; l < NT; ++l) cxx[l] = czz[l] = cxz[l] = cwin[l] = cuo[l] = cwo[l] = T(0);
-    // Coefficients of rows i .. i-NT, which the NT levels read.  Float
-    // kernels keep them in a per-wave LDS ring of NT+1 rows (row k in slot
-    // k mod (NT+1); a wave only touches its own slots, and its LDS accesses
-    // stay in program order, so no barrier is needed).  That frees 5 (NT+1)
-    // VGPRs: the production forward at NT 5 and adjoint at NT 5 reach 4 waves
-    // per SIMD, and the gradient got 11 ms faster.  Double kernels keep the
-    // ring in registers (their LDS ring would exceed 64 KB per workgroup).
-    constexpr bool kLds = std::is_same<T, float>::value;
-    constexpr int NC = B1 ? 6 : 5, NS = NT + 1;
-    __shared__ T cring_s[kLds ? kMarchWaves * NS * NC * kMarchLanes : 1];
-    T* const cwave = cring_s + (kLds ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * NS * NC * kMarchLanes + lane : 0);
-    CoefRow<T> ring[kLds ? 1 : NT + 1];
-    if constexpr (kLds) {
-#pragma unroll
-        for (int k = 0; k < NS * NC; ++k) cwave[k * kMarchLanes] = T(0);
-    } else {
-#pragma unroll
-        for (int k = 0; k <= NT; ++k) ring[k] = CoefRow<T>{T(0), T(0), T(0), T(0), T(0), T(0)};
-    }
+    __shared__ T cring_s[CoefRing<T, NT, B1>::kLdsElems];
+    CoefRing<T, NT, B1> ring;
+    ring.init(cring_s, __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane);
 
     const int i0 = z0 - NT - 1, i1 = z1 - 1 + NT;
     // row loads are issued one iteration ahead so their latency overlaps the
@@ -237,40 +284,14 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
     auto row = [&](const RowLoad<T, B1>& cur, const int i) {
         // level 0: row i of the stored state and its coefficients
         T ixx = cur.xx, izz = cur.zz, ixz = cur.xz, iu = cur.uu, iw = cur.ww;
-        CoefRow<T> cqv = cur.c;
-        if constexpr (kLds) {
-            T* w = cwave + ((i + 2 * NS) % NS) * NC * kMarchLanes;
-            w[0] = cur.c.temp;
-            w[kMarchLanes] = cur.c.b;
-            if (B1) w[2 * kMarchLanes] = cur.c.b1;
-            w[(NC - 3) * kMarchLanes] = cur.c.ca;
-            w[(NC - 2) * kMarchLanes] = cur.c.cl;
-            w[(NC - 1) * kMarchLanes] = cur.c.cm1;
-        } else {
-#pragma unroll
-            for (int k = NT; k > 0; --k) ring[k] = ring[k - 1];
-            ring[0] = cur.c;
-        }
+        ring.push(i, cur.c);
+        CoefRow<T> cq = cur.c;  // coefficients of level l's input row i - l
 
 #pragma unroll
         for (int l = 0; l < NT; ++l) {
             // level l+1 consumes level-l row q = i - l, produces row r = q - 1
             const int r = i - l - 1;
-            CoefRow<T> crv;
-            if constexpr (kLds) {
-                const T* rr = cwave + ((i - l - 1 + 2 * NS) % NS) * NC * kMarchLanes;
-                crv.temp = rr[0];
-                crv.b = rr[kMarchLanes];
-                crv.b1 = B1 ? rr[2 * kMarchLanes] : crv.b;
-                crv.ca = rr[(NC - 3) * kMarchLanes];
-                crv.cl = rr[(NC - 2) * kMarchLanes];
-                crv.cm1 = rr[(NC - 1) * kMarchLanes];
-            } else {
-                cqv = ring[l];
-                crv = ring[l + 1];
-            }
-            const CoefRow<T>& cq = cqv;
-            const CoefRow<T>& cr_ = crv;
+            const CoefRow<T> cr_ = ring.below(i, l);
             // uu_new(q)
             const T xx_r = tb_right(ixx);
             const T u_q = cell_vel(cq.temp, iu, cq.b, xx_r - ixx + ixz - cxz[l]);
@@ -322,7 +343,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
             cwo[l] = w_r;
             // level-(l+1) row r becomes the input of the next level
             ixx = oxx; izz = ozz; ixz = oxz; iu = u_r; iw = w_r;
-            if constexpr (kLds) cqv = crv;
+            cq = cr_;
         }
         // final level: row i - NT
         const int rf = i - NT;
@@ -459,67 +480,25 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
 
     // carried per level: previous input row (stress, old ww), previous
     // outputs u_l(r), w_l(r-1).  The previous row's coefficient products are
-    // formed again from it and its coefficients (ring[l + 1]): the same
+    // formed again from it and its coefficients (ring.below(i, l)): the same
     // products, without carrying 3 NT more registers.
     T cxx[NT], czz[NT], cxz[NT], cwin[NT], cuo[NT], cwo[NT];
 #pragma unroll
     for (int l = 0; l < NT; ++l) cxx[l] = czz[l] = cxz[l] = cwin[l] = cuo[l] = cwo[l] = T(0);
-    // Coefficients of rows i .. i-NT, which the NT levels read.  Float
-    // kernels keep them in a per-wave LDS ring of NT+1 rows (row k in slot
-    // k mod (NT+1); a wave only touches its own slots, and its LDS accesses
-    // stay in program order, so no barrier is needed).  That frees 5 (NT+1)
-    // VGPRs: the production forward at NT 5 and adjoint at NT 5 reach 4 waves
-    // per SIMD, and the gradient got 11 ms faster.  Double kernels keep the
-    // ring in registers (their LDS ring would exceed 64 KB per workgroup).
-    constexpr bool kLds = std::is_same<T, float>::value;
-    constexpr int NC = B1 ? 6 : 5, NS = NT + 1;
-    __shared__ T cring_s[kLds ? kMarchWaves * NS * NC * kMarchLanes : 1];
-    T* const cwave = cring_s + (kLds ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * NS * NC * kMarchLanes + lane : 0);
-    CoefRow<T> ring[kLds ? 1 : NT + 1];
-    if constexpr (kLds) {
-#pragma unroll
-        for (int k = 0; k < NS * NC; ++k) cwave[k * kMarchLanes] = T(0);
-    } else {
-#pragma unroll
-        for (int k = 0; k <= NT; ++k) ring[k] = CoefRow<T>{T(0), T(0), T(0), T(0), T(0), T(0)};
-    }
+    __shared__ T cring_s[CoefRing<T, NT, B1>::kLdsElems];
+    CoefRing<T, NT, B1> ring;
+    ring.init(cring_s, __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane);
 
     const int i0 = z0 - NT - 1, i1 = z1 - 1 + NT;
     auto row = [&](const RowLoad<T, B1>& cur, const ImgRow<T>& icur, const int i) {
         T ixx = cur.xx, izz = cur.zz, ixz = cur.xz, iu = cur.uu, iw = cur.ww;
-        CoefRow<T> cqv = cur.c;
-        if constexpr (kLds) {
-            T* w = cwave + ((i + 2 * NS) % NS) * NC * kMarchLanes;
-            w[0] = cur.c.temp;
-            w[kMarchLanes] = cur.c.b;
-            if (B1) w[2 * kMarchLanes] = cur.c.b1;
-            w[(NC - 3) * kMarchLanes] = cur.c.ca;
-            w[(NC - 2) * kMarchLanes] = cur.c.cl;
-            w[(NC - 1) * kMarchLanes] = cur.c.cm1;
-        } else {
-#pragma unroll
-            for (int k = NT; k > 0; --k) ring[k] = ring[k - 1];
-            ring[0] = cur.c;
-        }
+        ring.push(i, cur.c);
+        CoefRow<T> cq = cur.c;  // coefficients of level l's input row i - l
 
 #pragma unroll
         for (int l = 0; l < NT; ++l) {
             const int r = i - l - 1;
-            CoefRow<T> crv;
-            if constexpr (kLds) {
-                const T* rr = cwave + ((i - l - 1 + 2 * NS) % NS) * NC * kMarchLanes;
-                crv.temp = rr[0];
-                crv.b = rr[kMarchLanes];
-                crv.b1 = B1 ? rr[2 * kMarchLanes] : crv.b;
-                crv.ca = rr[(NC - 3) * kMarchLanes];
-                crv.cl = rr[(NC - 2) * kMarchLanes];
-                crv.cm1 = rr[(NC - 1) * kMarchLanes];
-            } else {
-                cqv = ring[l];
-                crv = ring[l + 1];
-            }
-            const CoefRow<T>& cq = cqv;
-            const CoefRow<T>& cr_ = crv;
+            const CoefRow<T> cr_ = ring.below(i, l);
             // coefficient products of input row q = r + 1
             const T caxx = cq.ca * ixx, clzz = cq.cl * izz, clxx = cq.cl * ixx, cazz = cq.ca * izz,
                     cmxz = cq.cm1 * ixz;
@@ -620,7 +599,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
             cuo[l] = u_q;
             cwo[l] = w_r;
             ixx = oxx; izz = ozz; ixz = oxz; iu = u_r; iw = w_r;
-            if constexpr (kLds) cqv = crv;
+            cq = cr_;
         }
         const int rf = i - NT;
         if (rf >= z0 && rf < z1 && out) {
