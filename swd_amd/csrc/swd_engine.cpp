@@ -460,26 +460,19 @@ static float elapsed(hipEvent_t a, hipEvent_t b)
 
 int Engine::gradient(double* dk_vs, double* res_r, double* res_l, double* cr_r, double* cr_l)
 {
-    hipEvent_t e[5];
-    for (auto& x : e) SWD_HIP(hipEventCreate(&x));
+    EventSet ev(5);
     int rc;
-    SWD_HIP(hipEventRecord(e[0], st));
-    if ((rc = forward(true))) goto done;
-    SWD_HIP(hipEventRecord(e[1], st));
-    if ((rc = dispersion()) || (rc = adjoint_source())) goto done;
-    SWD_HIP(hipEventRecord(e[2], st));
-    if ((rc = adjoint())) goto done;
-    SWD_HIP(hipEventRecord(e[3], st));
-    if ((rc = assemble())) goto done;
-    SWD_HIP(hipEventRecord(e[4], st));
+    if ((rc = ev.create()) || (rc = ev.record(0, st)) || (rc = forward(true)) || (rc = ev.record(1, st)) ||
+        (rc = dispersion()) || (rc = adjoint_source()) || (rc = ev.record(2, st)) || (rc = adjoint()) ||
+        (rc = ev.record(3, st)) || (rc = assemble()) || (rc = ev.record(4, st)))
+        return rc;
     if (dk_vs) SWD_HIP(hipMemcpyAsync(dk_vs, outs.p, splane * sizeof(double), hipMemcpyDeviceToHost, st));
     rc = read_picks(res_r, res_l, cr_r, cr_l);
+    const hipEvent_t* e = ev.ev.data();
     times.fwd_ms = elapsed(e[0], e[1]);
     times.disp_ms = elapsed(e[1], e[2]);
     times.adj_ms = elapsed(e[2], e[3]);
     times.asm_ms = elapsed(e[3], e[4]);
-done:
-    for (auto& x : e) (void)hipEventDestroy(x);
     return rc;
 }
 

@@ -157,6 +157,29 @@ struct Stream {
     }
 };
 
+// Timing events, destroyed on every return path.
+struct EventSet {
+    std::vector<hipEvent_t> ev;
+    explicit EventSet(size_t n) : ev(n, nullptr) {}
+    EventSet(const EventSet&) = delete;
+    EventSet& operator=(const EventSet&) = delete;
+    ~EventSet()
+    {
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    int create()
+    {
+        for (auto& e : ev) SWD_HIP(hipEventCreate(&e));
+        return kOk;
+    }
+    int record(size_t i, hipStream_t st)
+    {
+        SWD_HIP(hipEventRecord(ev[i], st));
+        return kOk;
+    }
+};
+
 struct MarchPlan {
     std::vector<BlockMap> bmap;
     int nblocks = 0;

@@ -152,43 +152,43 @@ int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int steps, 
     const int ndom = nL > 0 ? nL : rem;
     const int stride = nL > 0 ? 7 : 37;
     const int nsamp = (ndom + stride - 1) / stride;
-    std::vector<hipEvent_t> ev(2 * nsamp + 2);
-    for (auto& e : ev) SWD_HIP(hipEventCreate(&e));
-    rc = kOk;
+    EventSet ev(2 * nsamp + 2);
+    if ((rc = ev.create())) return rc;
+    // from the first launch on, errors only stop the loops: the stream is
+    // drained before the tables above are released
     int launch = 0;
-    SWD_HIP(hipEventRecord(ev[2 * nsamp], st));
+    rc = ev.record(2 * nsamp, st);
     for (size_t k = 0; k < segs.size() && rc == kOk; ++k) {
         const TbSegment<T>& sg = segs[k];
         for (int L = sg.L0; L < sg.L1 && rc == kOk; ++L, ++launch) {
             const bool samp = L % stride == 0;
-            if (samp) SWD_HIP(hipEventRecord(ev[2 * (L / stride)], st));
+            if (samp && (rc = ev.record(2 * (L / stride), st))) break;
             rc = adj ? launch_adj_tb<T>(st, djt[k].as<FdJob<T>>(), dbt[k].as<BlockMap>(), sg.plan.nblocks, L * NT,
                                         launch, NT, b1)
                      : launch_fwd_tb<T>(st, djt[k].as<FdJob<T>>(), dbt[k].as<BlockMap>(), sg.plan.nblocks, L * NT,
                                         launch, NT, b1);
-            if (samp && rc == kOk) SWD_HIP(hipEventRecord(ev[2 * (L / stride) + 1], st));
+            if (samp && rc == kOk) rc = ev.record(2 * (L / stride) + 1, st);
         }
     }
     launch = nL;
     for (int j = 0; j < rem && rc == kOk; ++j, ++launch) {
         const bool samp = nL == 0 && j % stride == 0;
-        if (samp) SWD_HIP(hipEventRecord(ev[2 * (j / stride)], st));
+        if (samp && (rc = ev.record(2 * (j / stride), st))) break;
         const int step = nL * NT + j;
         rc = adj ? launch_adj_march<T>(st, djs.as<FdJob<T>>(), dbs.as<BlockMap>(), pss.nblocks, step, launch)
                  : launch_fwd_march<T>(st, djs.as<FdJob<T>>(), dbs.as<BlockMap>(), pss.nblocks, step, launch);
-        if (samp && rc == kOk) SWD_HIP(hipEventRecord(ev[2 * (j / stride) + 1], st));
+        if (samp && rc == kOk) rc = ev.record(2 * (j / stride) + 1, st);
     }
-    if (rc == kOk) SWD_HIP(hipEventRecord(ev[2 * nsamp + 1], st));
+    if (rc == kOk) rc = ev.record(2 * nsamp + 1, st);
     const hipError_t se = hipStreamSynchronize(st);
     float tot = 0, span = 0;
     if (rc == kOk) {
         for (int i = 0; i < nsamp; ++i) {
             float ms = 0;
-            if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) == hipSuccess) tot += ms;
+            if (hipEventElapsedTime(&ms, ev.ev[2 * i], ev.ev[2 * i + 1]) == hipSuccess) tot += ms;
         }
-        if (hipEventElapsedTime(&span, ev[2 * nsamp], ev[2 * nsamp + 1]) != hipSuccess) span = 0;
+        if (hipEventElapsedTime(&span, ev.ev[2 * nsamp], ev.ev[2 * nsamp + 1]) != hipSuccess) span = 0;
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
     if (rc) return rc;
     if (se != hipSuccess) { set_error("FD march failed: %s", hipGetErrorString(se)); return kErrHip; }
     if (stats) {
@@ -287,12 +287,17 @@ int run_march_graph(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int s
         g.nt_block = nL > 0 ? NT : 1;
         g.rows = nL > 0 ? segs[0].rows : ss_rows;
     }
-    SWD_HIP(hipEventRecord(g.ev[0], st));
-    if (g.exec[0]) SWD_HIP(hipGraphLaunch(g.exec[0], st));
-    SWD_HIP(hipEventRecord(g.ev[1], st));
-    if (g.exec[1]) SWD_HIP(hipGraphLaunch(g.exec[1], st));
-    SWD_HIP(hipEventRecord(g.ev[2], st));
-    SWD_HIP(hipStreamSynchronize(st));
+    // a failed enqueue still drains what went before it
+    hipError_t qe = hipEventRecord(g.ev[0], st);
+    for (int part = 0; part < 2 && qe == hipSuccess; ++part) {
+        if (g.exec[part]) qe = hipGraphLaunch(g.exec[part], st);
+        if (qe == hipSuccess) qe = hipEventRecord(g.ev[part + 1], st);
+    }
+    const hipError_t se = hipStreamSynchronize(st);
+    if (qe != hipSuccess || se != hipSuccess) {
+        set_error("FD graph replay failed: %s", hipGetErrorString(qe != hipSuccess ? qe : se));
+        return kErrHip;
+    }
     float span[2] = {0, 0};
     for (int part = 0; part < 2; ++part)
         if (hipEventElapsedTime(&span[part], g.ev[part], g.ev[part + 1]) != hipSuccess) span[part] = 0;
