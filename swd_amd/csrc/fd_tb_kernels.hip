@@ -490,8 +490,12 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
     ring.init(cring_s, __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane);
 
     const int i0 = z0 - NT - 1, i1 = z1 - 1 + NT;
-    auto row = [&](const RowLoad<T, B1>& cur, const ImgRow<T>& icur, const int i) {
+    // lean: at most one imaging level in this launch, imaged after the level
+    // chain from prefetched values; otherwise every imaging level images inline
+    auto row = [&](const RowLoad<T, B1>& cur, const ImgRow<T>& icur, const int i, auto lean) {
+        constexpr bool kLean = decltype(lean)::value;
         T ixx = cur.xx, izz = cur.zz, ixz = cur.xz, iu = cur.uu, iw = cur.ww;
+        T sxx = T(0), szz = T(0), sxz = T(0);
         ring.push(i, cur.c);
         CoefRow<T> cq = cur.c;  // coefficients of level l's input row i - l
 
@@ -519,83 +523,48 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
             const T oxx = cell_adj_stress(tc, cxx[l], dtx, u_rt - u_r);
             const T ozz = (r == pt) ? T(0) : cell_adj_stress(tc, czz[l], dtx, w_r - cwo[l]);
             const T oxz = cell_adj_shear(tc, cxz[l], dtx, u_q - u_r, w_r - w_l);
-            if (pf_img) {
-                if (l == limg && r >= z0 && r < z1 && r > pt && r <= pt + nzi && icol >= 0) {
-                    // prefetched snapshot and image values of this row
-                    double c0, c1, c2;
-                    image_terms(icur.fux, icur.fwz, icur.fsh, oxx, ozz, oxz, c0, c1, c2);
+            if constexpr (kLean) {
+                // the imaging level's adjoint stresses, imaged after the chain
+                if (l == limg) { sxx = oxx; szz = ozz; sxz = oxz; }
+            } else {
+                if (itl[l] >= 0 && (itl[l] % in_wf) == 0 && r >= z0 && r < z1 && r > pt && r <= pt + nzi &&
+                           icol >= 0) {
+                    const size_t plane = (size_t)nzi * nxi;
                     const size_t io = (size_t)(r - pt - 1) * nxi + icol;
+                    const gptr<const T> sb = snap + (size_t)(itl[l] / in_wf) * nsf * plane;
+                    const gptr<const T> sn = sb + io;
+                    double Fux, Fwz, Fsh;
+                    if (nsf == 3) {
+                        const gptr<const T> sc = sb + (size_t)(r - pt - 1) * 3 * nxi + icol;
+                        Fux = stream_ld(sc); Fwz = stream_ld(sc + nxi); Fsh = stream_ld(sc + 2 * nxi);
+                    } else {
+                        Fux = stream_ld(sn); Fwz = stream_ld(sn + 3 * plane);
+                        Fsh = (double)(stream_ld(sn + plane) + stream_ld(sn + 2 * plane));
+                    }
+                    double c0, c1, c2;
+                    image_terms(Fux, Fwz, Fsh, oxx, ozz, oxz, c0, c1, c2);
                     const gptr<double> im = img + io;
+                    const gptr<const double> ic = im;
                     if constexpr (kPart) {
                         const gptr<T> pp = imgp + 3 * io;
-                        if (flush) {
-                            // image_flush as atomic adds without return: the same sum
-                            // (one lane per cell and launch), and no wait on an FP64
-                            // load (flush launches 162 -> 138 us, the others 108-110)
-                            double* imd = (double*)im;
-                            atomicAdd(imd, (double)icur.m0 + c0);
-                            atomicAdd(imd + iplane, (double)icur.m1 + c1);
-                            atomicAdd(imd + 2 * iplane, (double)icur.m2 + c2);
+                        const gptr<const T> pc = pp;
+                        if (((itl[l] / in_wf) % kImgFlush) == 0) {
+                            stream_st(im, image_flush(stream_ld(ic), stream_ld(pc), c0));
+                            stream_st(im + plane, image_flush(stream_ld(ic + plane), stream_ld(pc + 1), c1));
+                            stream_st(im + 2 * plane, image_flush(stream_ld(ic + 2 * plane), stream_ld(pc + 2), c2));
                             stream_st(pp, T(0));
                             stream_st(pp + 1, T(0));
                             stream_st(pp + 2, T(0));
                         } else {
-                            // image_partial for the three images, with one test on the common path
-                            double v0 = (double)icur.m0 + c0, v1 = (double)icur.m1 + c1, v2 = (double)icur.m2 + c2;
-                            // a spill is the only access to that FP64 image in this launch (one
-                            // imaging level per launch here), so it goes as an atomic add without
-                            // return: no wait for it on the path
-                            double* imd = (double*)im;
-                            if (image_tiny(v0)) { atomicAdd(imd, v0); v0 = 0.0; }
-                            if (image_tiny(v1)) { atomicAdd(imd + iplane, v1); v1 = 0.0; }
-                            if (image_tiny(v2)) { atomicAdd(imd + 2 * iplane, v2); v2 = 0.0; }
-                            stream_st(pp, (float)v0);
-                            stream_st(pp + 1, (float)v1);
-                            stream_st(pp + 2, (float)v2);
+                            stream_st(pp, image_partial(stream_ld(pc), c0, im));
+                            stream_st(pp + 1, image_partial(stream_ld(pc + 1), c1, im + plane));
+                            stream_st(pp + 2, image_partial(stream_ld(pc + 2), c2, im + 2 * plane));
                         }
                     } else {
-                        stream_st(im, (double)icur.m0 + c0);
-                        stream_st(im + iplane, (double)icur.m1 + c1);
-                        stream_st(im + 2 * iplane, (double)icur.m2 + c2);
+                        stream_st(im, stream_ld(ic) + c0);
+                        stream_st(im + plane, stream_ld(ic + plane) + c1);
+                        stream_st(im + 2 * plane, stream_ld(ic + 2 * plane) + c2);
                     }
-                }
-            } else if (itl[l] >= 0 && (itl[l] % in_wf) == 0 && r >= z0 && r < z1 && r > pt && r <= pt + nzi &&
-                       icol >= 0) {
-                const size_t plane = (size_t)nzi * nxi;
-                const size_t io = (size_t)(r - pt - 1) * nxi + icol;
-                const gptr<const T> sb = snap + (size_t)(itl[l] / in_wf) * nsf * plane;
-                const gptr<const T> sn = sb + io;
-                double Fux, Fwz, Fsh;
-                if (nsf == 3) {
-                    const gptr<const T> sc = sb + (size_t)(r - pt - 1) * 3 * nxi + icol;
-                    Fux = stream_ld(sc); Fwz = stream_ld(sc + nxi); Fsh = stream_ld(sc + 2 * nxi);
-                } else {
-                    Fux = stream_ld(sn); Fwz = stream_ld(sn + 3 * plane);
-                    Fsh = (double)(stream_ld(sn + plane) + stream_ld(sn + 2 * plane));
-                }
-                double c0, c1, c2;
-                image_terms(Fux, Fwz, Fsh, oxx, ozz, oxz, c0, c1, c2);
-                const gptr<double> im = img + io;
-                const gptr<const double> ic = im;
-                if constexpr (kPart) {
-                    const gptr<T> pp = imgp + 3 * io;
-                    const gptr<const T> pc = pp;
-                    if (((itl[l] / in_wf) % kImgFlush) == 0) {
-                        stream_st(im, image_flush(stream_ld(ic), stream_ld(pc), c0));
-                        stream_st(im + plane, image_flush(stream_ld(ic + plane), stream_ld(pc + 1), c1));
-                        stream_st(im + 2 * plane, image_flush(stream_ld(ic + 2 * plane), stream_ld(pc + 2), c2));
-                        stream_st(pp, T(0));
-                        stream_st(pp + 1, T(0));
-                        stream_st(pp + 2, T(0));
-                    } else {
-                        stream_st(pp, image_partial(stream_ld(pc), c0, im));
-                        stream_st(pp + 1, image_partial(stream_ld(pc + 1), c1, im + plane));
-                        stream_st(pp + 2, image_partial(stream_ld(pc + 2), c2, im + 2 * plane));
-                    }
-                } else {
-                    stream_st(im, stream_ld(ic) + c0);
-                    stream_st(im + plane, stream_ld(ic + plane) + c1);
-                    stream_st(im + 2 * plane, stream_ld(ic + 2 * plane) + c2);
                 }
             }
             cxx[l] = ixx; czz[l] = izz; cxz[l] = ixz; cwin[l] = iw;
@@ -603,6 +572,48 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
             cwo[l] = w_r;
             ixx = oxx; izz = ozz; ixz = oxz; iu = u_r; iw = w_r;
             cq = cr_;
+        }
+        if constexpr (kLean) {
+            const int r = i - limg - 1;
+            if (pf_img && r >= z0 && r < z1 && r > pt && r <= pt + nzi && icol >= 0) {
+                // prefetched snapshot and image values of this row
+                double c0, c1, c2;
+                image_terms(icur.fux, icur.fwz, icur.fsh, sxx, szz, sxz, c0, c1, c2);
+                const size_t io = (size_t)(r - pt - 1) * nxi + icol;
+                const gptr<double> im = img + io;
+                if constexpr (kPart) {
+                    const gptr<T> pp = imgp + 3 * io;
+                    if (flush) {
+                        // image_flush as atomic adds without return: the same sum
+                        // (one lane per cell and launch), and no wait on an FP64
+                        // load (flush launches 162 -> 138 us, the others 108-110)
+                        double* imd = (double*)im;
+                        atomicAdd(imd, (double)icur.m0 + c0);
+                        atomicAdd(imd + iplane, (double)icur.m1 + c1);
+                        atomicAdd(imd + 2 * iplane, (double)icur.m2 + c2);
+                        stream_st(pp, T(0));
+                        stream_st(pp + 1, T(0));
+                        stream_st(pp + 2, T(0));
+                    } else {
+                        // image_partial for the three images, with one test on the common path
+                        double v0 = (double)icur.m0 + c0, v1 = (double)icur.m1 + c1, v2 = (double)icur.m2 + c2;
+                        // a spill is the only access to that FP64 image in this launch (one
+                        // imaging level per launch here), so it goes as an atomic add without
+                        // return: no wait for it on the path
+                        double* imd = (double*)im;
+                        if (image_tiny(v0)) { atomicAdd(imd, v0); v0 = 0.0; }
+                        if (image_tiny(v1)) { atomicAdd(imd + iplane, v1); v1 = 0.0; }
+                        if (image_tiny(v2)) { atomicAdd(imd + 2 * iplane, v2); v2 = 0.0; }
+                        stream_st(pp, (float)v0);
+                        stream_st(pp + 1, (float)v1);
+                        stream_st(pp + 2, (float)v2);
+                    }
+                } else {
+                    stream_st(im, (double)icur.m0 + c0);
+                    stream_st(im + iplane, (double)icur.m1 + c1);
+                    stream_st(im + 2 * iplane, (double)icur.m2 + c2);
+                }
+            }
         }
         const int rf = i - NT;
         if (rf >= z0 && rf < z1 && out) {
@@ -616,19 +627,24 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
     };
     // Rows are loaded one step ahead into two alternating buffers, so the
     // loads of row i+1 are in flight while row i is processed.
-    RowLoad<T, B1> A, B;
-    ImgRow<T> IA, IB;
-    A.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i0, nzbc, rowb, xb);
-    img_load(IA, i0);
-    for (int i = i0; i <= i1; i += 2) {
-        B.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i + 1, nzbc, rowb, xb);
-        img_load(IB, i + 1);
-        row(A, IA, i);
-        if (i + 1 > i1) break;
-        A.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i + 2, nzbc, rowb, xb);
-        img_load(IA, i + 2);
-        row(B, IB, i + 1);
-    }
+    auto march = [&](auto lean) {
+        constexpr bool kLean = decltype(lean)::value;
+        RowLoad<T, B1> A, B;
+        ImgRow<T> IA, IB;
+        A.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i0, nzbc, rowb, xb);
+        if constexpr (kLean) img_load(IA, i0);
+        for (int i = i0; i <= i1; i += 2) {
+            B.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i + 1, nzbc, rowb, xb);
+            if constexpr (kLean) img_load(IB, i + 1);
+            row(A, IA, i, lean);
+            if (i + 1 > i1) break;
+            A.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i + 2, nzbc, rowb, xb);
+            if constexpr (kLean) img_load(IA, i + 2);
+            row(B, IB, i + 1, lean);
+        }
+    };
+    if (nimg <= 1) march(std::true_type{});
+    else march(std::false_type{});
 }
 
 template <typename T, bool B1>
