@@ -145,3 +145,31 @@ def test_mex_separate_b1_plane(case):
     for nt_block in (4, 5):
         for x, y in zip(run_adj(nt_block), g1):
             np.testing.assert_array_equal(x, y)
+
+
+def test_engine_gradient_bitwise_with_several_imaging_levels_per_launch():
+    """Snapshots every 2 steps: a 5-step launch holds 2 or 3 snapshot and
+    imaging levels, so the kernels take their general (inline) record and
+    imaging paths instead of the one-level ones."""
+    prob, mod = P.plain_single_shot(nt=800)
+    prob = dict(prob, in_wf=2)
+    e = Engine(prob)
+    try:
+        e.set_wavelet(mod["s"])
+        e.set_model(mod["vp_true"], mod["vs_true"])
+        e.observe()
+        e.set_model(mod["vp"], mod["vs"])
+        e.set_time_block(1, 32)
+        g1 = e.gradient()
+        gath1 = e.read("gathers")
+        # cells the short record never reaches have zero illumination (0/0 there, as in the reference)
+        lit = np.isfinite(g1["dk_vs"])
+        assert lit.mean() > 0.3 and np.any(g1["dk_vs"][lit] != 0)
+        for nt_block, rows in ((5, 16), (3, 0), (2, 24)):
+            e.set_time_block(nt_block, rows)
+            g = e.gradient()
+            assert e.timings()["adj_steps_per_launch"] == nt_block
+            np.testing.assert_array_equal(e.read("gathers"), gath1)
+            np.testing.assert_array_equal(g["dk_vs"], g1["dk_vs"])
+    finally:
+        e.close()
