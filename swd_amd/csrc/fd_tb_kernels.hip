@@ -586,7 +586,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
                     if (flush) {
                         // image_flush as atomic adds without return: the same sum
                         // (one lane per cell and launch), and no wait on an FP64
-                        // load (flush launches 162 -> 138 us, the others 108-110)
+                        // load (24 us less per flush launch at the production shape)
                         double* imd = (double*)im;
                         atomicAdd(imd, (double)icur.m0 + c0);
                         atomicAdd(imd + iplane, (double)icur.m1 + c1);
