@@ -130,6 +130,28 @@ def test_outcrop_manual_cube_fits_the_outcrop_problem():
         assert np.all(np.isfinite(cr)) and cr.min() >= 1 and cr.max() <= 1000
 
 
+GOLDEN_PICKS = os.path.join(os.path.dirname(__file__), "golden", "outcrop_manual_picks_offset50.npz")
+
+
+def test_outcrop_manual_picks_fixture():
+    """The committed fixture is the outcrop script's slice of its manual cubes
+    (regenerated and compared where outcropDisp.mat is present)."""
+    d = np.load(GOLDEN_PICKS)
+    prob, _ = F.outcrop(offset=int(d["offset"]))
+    from swd_amd.engine import problem_npair
+    assert d["cr0_r"].shape == d["cr0_l"].shape == (problem_npair(prob), d["shots"].size)
+    assert np.all(np.isfinite(d["cr0_r"])) and np.all(np.isfinite(d["cr0_l"]))
+    if os.path.exists(OUTCROP_DISP):
+        import importlib.util
+        spec = importlib.util.spec_from_file_location(
+            "make_outcrop_manual_picks", os.path.join(os.path.dirname(GOLDEN_PICKS), "make_outcrop_manual_picks.py"))
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        fresh = mod.make(OUTCROP_DISP)
+        for k in d.files:
+            np.testing.assert_array_equal(d[k], fresh[k])
+
+
 def _fk_brute(d):
     """fk_filter.m's steps with explicit sums and explicit shift indices (1-based as in MATLAB)."""
     ns, ntr = d.shape
