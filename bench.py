@@ -376,7 +376,7 @@ def main():
     ap.add_argument("--no-field", action="store_true", help="skip the outcrop-survey leg")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): this build has no multi-GPU "
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): libswd has no multi-GPU "
               "path (shots would split across ranks with one image all-reduce per gradient)", file=sys.stderr)
         sys.exit(2)
 
