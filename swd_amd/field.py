@@ -129,3 +129,39 @@ def outcrop(offset: int = 50, nshots: int | None = None, nt: int | None = None):
                 fmax=85.0, vmin=100.0, vmax=800.0, ds=4 * dx, offset=offset, m=0, M=1, ini=200, w=4.0, beta=1e4,
                 ns=ns, is_index=np.arange(1, nshots + 1))
     return prob, {"vs": vs, "vp": vp, "s": ricker(30.0, dt, nt)}
+
+
+# --------------------------------------------------------------------------
+# The Qadimah-fault survey (fieldexamples/QadimahFault/QademahSWD3.m:75-136)
+# --------------------------------------------------------------------------
+
+QADIMAH_VS_BOUNDS = (100.0, 1200.0)        # vsmin, vsmax (QademahSWD3.m:76)
+
+
+def qadimah(offset: int = 39, nshots: int | None = None, nt: int | None = None):
+    """The Qadimah-fault survey: 120 x 2400 cells at 0.25 m, 40-cell borders,
+    120 shots every 20 cells and 240 receivers every 10 cells (2.5 m traces,
+    M = 2), 30 Hz Ricker, dt = dx/vpmax/2 with vpmax = 1.732 * 1200, nt 8500,
+    snapshots every 10 steps (dt_wf = 10 dt); Radon 100-1000 m/s at df = 0.1 Hz
+    over 10-60 Hz, windows of ``offset`` traces, w = 3, SoftArgNorm 1e4.
+    Initial model vs = linspace(200, 1100) with depth, vp = 1.732 vs
+    (QademahSWD3.m:75-123).  The pick starts at the strongest column (the
+    "qadimah" variant), so ``ini`` is not used.  ``nshots`` keeps the first
+    shots and ``nt`` overrides the trace length (shorter test runs); ns of the
+    branch guards stays 120."""
+    vsmax = 1200.0
+    vpmax = vsmax * 1.732
+    dx = 0.25
+    dt = dx / vpmax * 0.5
+    nz, nx, nbc = 120, 2400, 40
+    vs = np.tile(np.linspace(200, 1100, nz)[:, None], (1, nx))
+    vp = vs * 1.732
+    nt = 8500 if nt is None else int(nt)
+    sx = np.arange(1, nx + 1, 20)
+    ns = sx.size
+    nshots = ns if nshots is None else int(nshots)
+    prob = dict(nz=nz, nx=nx, nbc=nbc, fd_order=22, nt=nt, in_wf=10, dx=dx, dt=dt, sx=sx[:nshots],
+                sz=np.ones(nshots, dtype=int), ng=240, gx0=1, dg=10, gz=1, df=0.1, fmin=10.0,
+                fmax=60.0, vmin=100.0, vmax=1000.0, ds=10 * dx, offset=offset, m=0, M=2, ini=1, w=3.0, beta=1e4,
+                ns=ns, is_index=np.arange(1, nshots + 1))
+    return prob, {"vs": vs, "vp": vp, "s": ricker(30.0, dt, nt)}

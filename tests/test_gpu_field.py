@@ -216,6 +216,81 @@ def test_outcrop_gradient_with_the_scripts_manual_picks_vs_oracle():
     assert ed < 1e-4
 
 
+def test_qadimah_gradient_with_the_scripts_manual_picks_vs_oracle():
+    """Shot 60 of the Qadimah-fault survey (162 x 2480 padded, 240 receivers
+    every 10 cells, M = 2, snapshots every 10 steps) at offset 39 with the
+    script's smoothed manual picks (a fixture cut from manualDispfile3of3.mat)
+    and the "qadimah" variant, nt cut to 2000 (the FP64 oracle needs about a
+    minute for this shot): picks within one bin of the oracle's, misfits equal
+    where the picks are, and, given the oracle's picks, adjoint sources and
+    dk_vs within 1e-4 of FP64."""
+    import os
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "qadimah_manual_picks_offset39.npz"))
+    idx = np.array([60])
+    cols = [int(np.flatnonzero(fx["shots"] == s)[0]) for s in idx]
+    prob, mod = F.qadimah(offset=int(fx["offset"]), nt=2000)
+    q = prob
+    q["sx"] = 1 + 20 * (idx - 1)
+    q["sz"] = np.ones(idx.size, dtype=int)
+    q["is_index"] = idx
+    assert float(fx["fmin"]) == q["fmin"] and float(fx["fmax"]) == q["fmax"] and float(fx["df"]) == q["df"]
+    cr0_r, cr0_l = fx["cr0_r"][:, cols], fx["cr0_l"][:, cols]
+    var = dict(F.VARIANTS["qadimah"])
+    gx = q["gx0"] + q["dg"] * np.arange(q["ng"], dtype=float)
+    gz = np.full(q["ng"], float(q["gz"]))
+    radon = dict(df=q["df"], fmin=q["fmin"], fmax=q["fmax"], vmin=q["vmin"], vmax=q["vmax"], ds=q["ds"],
+                 offset=q["offset"], m=q["m"], M=q["M"])
+    e = Engine(prob)
+    try:
+        e.set_variant("qadimah")
+        e.set_wavelet(mod["s"])
+        e.set_model(mod["vp"], mod["vs"])
+        e.set_observed_picks(cr0_r, cr0_l)
+        gpu = e.gradient()
+        gath = e.read("gathers")
+        shots = []
+        for k, is_ in enumerate(q["is_index"]):
+            o = O.swd_shot(mod["vp"], mod["vs"], mod["s"], q["nbc"], q["nt"], q["dx"], q["dt"], q["sx"][k],
+                           q["sz"][k], gx, gz, q["in_wf"], 22, int(is_), q["ns"], radon, cr0_r[:, k], cr0_l[:, k],
+                           q["beta"], q["ini"], q["w"], variant=var)
+            shots.append(o)
+            eg = rel(gath[k], o["seismo_w"])
+            dr = np.max(np.abs(gpu["cr_r"][:, k] - o["cr_r"]))
+            dl = np.max(np.abs(gpu["cr_l"][:, k] - o["cr_l"]))
+            print(f"shot {is_}: gathers rel-L2 {eg:.2e}, misfit right {gpu['res_r'][k]:.4g} "
+                  f"(oracle {o['res_r']:.4g}), pick differences right {dr} left {dl}")
+            assert eg < 1e-4 and dr <= 1 and dl <= 1
+            if dr == 0:
+                assert gpu["res_r"][k] == pytest.approx(o["res_r"], rel=1e-9)
+            if dl == 0:
+                assert gpu["res_l"][k] == pytest.approx(o["res_l"], rel=1e-9)
+        e.run("forward")
+        e.run("dispersion")
+        e.set_picks(np.array([o["cr_r"] for o in shots]).T, np.array([o["cr_l"] for o in shots]).T)
+        e.run("adjoint_source")
+        e.run("adjoint")
+        e.run("assemble")
+        adj = e.read("adjoint_sources")
+        dk_gpu = e.read("gradient")[0]
+    finally:
+        e.close()
+    for k, o in enumerate(shots):
+        ea = rel(adj[k], o["adj"])
+        print(f"shot {q['is_index'][k]}: adjoint source rel-L2 {ea:.2e}")
+        assert ea < 1e-4
+    imgs = sum(np.array([o["cl_img"], o["cm_img"], o["illum"]]) for o in shots)
+    dk_ref = O.assemble_dk_vs(mod["vs"], *imgs)
+    # most of the 600 m line is never reached in 2000 steps: compare where the
+    # FP64 illumination is above 1e-60 (below that FP32 strains read 0 and the
+    # ratio is 0/0, as in the reference's _single MEX)
+    lit = np.asarray(imgs[2]) > 1e-60
+    g, r = np.asarray(dk_gpu)[lit], np.asarray(dk_ref)[lit]
+    assert lit.sum() > 1000 and np.all(np.isfinite(g))
+    ed = float(np.linalg.norm(g - r) / np.linalg.norm(r))
+    print(f"dk_vs rel-L2 vs FP64 over {lit.sum()} lit cells: {ed:.2e}")
+    assert ed < 1e-4
+
+
 def test_iteration_with_vp_tied_to_vs_equals_host_restatement():
     """swd_engine_iterate with vp_ratio 1.732 (and the rest of the outcrop
     variant) against the oracle's line search driving a second engine whose

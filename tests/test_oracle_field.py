@@ -152,6 +152,33 @@ def test_outcrop_manual_picks_fixture():
             np.testing.assert_array_equal(d[k], fresh[k])
 
 
+QADIMAH_MANUAL = "/root/reference/fieldexamples/QadimahFault/manualDispfile3of3.mat"
+GOLDEN_QPICKS = os.path.join(os.path.dirname(__file__), "golden", "qadimah_manual_picks_offset39.npz")
+
+
+def test_qadimah_survey_and_manual_picks_fixture():
+    """QademahSWD3.m:75-136 geometry, and the committed fixture as the script's
+    smoothed slice of its manual cubes (regenerated and compared where
+    manualDispfile3of3.mat is present)."""
+    from swd_amd.engine import problem_npair
+    prob, mod = F.qadimah()
+    assert prob["sx"].size == prob["ns"] == 120 and prob["ng"] == 240 and prob["M"] == 2
+    assert prob["dt"] == pytest.approx(0.25 / (1200 * 1.732) * 0.5, rel=1e-15) and prob["nt"] % prob["in_wf"] == 0
+    assert problem_npair(prob) == 501 and mod["vs"][0, 0] == 200 and mod["vs"][-1, 7] == 1100
+    d = np.load(GOLDEN_QPICKS)
+    assert d["cr0_r"].shape == d["cr0_l"].shape == (501, d["shots"].size)
+    assert np.all(np.isfinite(d["cr0_r"])) and np.all(np.isfinite(d["cr0_l"]))
+    if os.path.exists(QADIMAH_MANUAL):
+        import importlib.util
+        spec = importlib.util.spec_from_file_location(
+            "make_qadimah_manual_picks", os.path.join(os.path.dirname(GOLDEN_QPICKS), "make_qadimah_manual_picks.py"))
+        m = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(m)
+        fresh = m.make(QADIMAH_MANUAL)
+        for k in d.files:
+            np.testing.assert_array_equal(d[k], fresh[k])
+
+
 def _fk_brute(d):
     """fk_filter.m's steps with explicit sums and explicit shift indices (1-based as in MATLAB)."""
     ns, ntr = d.shape
