@@ -39,10 +39,15 @@ LIB_PATH = os.path.join(HERE, "_build", "libswd_oracle.so")
 PARITY = {
     "oracle_prep": "hand-computed tiny cases (pad/calparam/damp_circle values, free-surface row edits)",
     "oracle_ricker": "closed form at the centre sample, symmetry, length rule",
-    "oracle_staggerfd": "two-step closed form after an impulse, linearity in s, zero source, "
+    "oracle_staggerfd": "two-step closed form after an impulse (order 22) and one-step closed form with "
+                        "S41/S42 (order 24: strains and the next velocities), linearity in s, zero source, "
                         "density invariance, receiver-on-source, zz free-surface row, FP32 twin",
-    "oracle_e2drtm": "one-step closed form of the first imaging contribution, illum closed form, "
-                     "linearity in the adjoint source, bilinearity in the snapshots",
+    "oracle_e2drtm": "closed forms read back through the imaging condition: the first stresses after one "
+                     "adjoint impulse (orders 22 and 24), the second step's coefficient-weighted velocity "
+                     "half-step derived cell by cell (order 22, also on the production receiver row), and "
+                     "three steps against a scatter form of the velocity half-step (orders 22 and 24, "
+                     "xx_b included); illum closed form, linearity in the adjoint source, bilinearity in "
+                     "the snapshots",
     "rt_side": "plane-wave peak at c0, single-trace constant |mm|, linearity, window/flip rules",
     "pickFDC1": "hand-built columns (single peak, width-jump carry-over + interpolation, ties)",
     "findpeaks": "hand-built vectors incl. plateaus and endpoints (published MATLAB rules)",

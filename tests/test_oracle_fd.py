@@ -311,3 +311,260 @@ def test_higher_orders_linear_and_zero(order):
     assert rel(m, a + 2 * b) < 1e-12
     _, z, _ = O.staggerfd(iv, c, np.zeros(20))
     assert not np.any(z)
+
+
+# ---------------------------------------------------------------- adjoint: the velocity half-step
+# The imaging condition reads the adjoint stresses back: with unit snapshots in
+# one slot and zeros elsewhere, cm_img = 2 XX (fux = 1), 2 ZZ (bwz = 1) or XZ
+# (fuz = 1), and cl_img = XX + ZZ (fux = 1), where XX, ZZ, XZ are the adjoint
+# stresses of that slot's step over the interior.
+S41, S42 = float(np.float32(1.1250)), float(np.float32(-0.0416666667))
+
+
+def _adj_stress_readout(c, iv, A, nrec, slot, nz, nx):
+    out = {}
+    for name in ("fux", "bwz", "fuz"):
+        sn = {k: np.zeros((nz, nx, nrec), order="F") for k in ("fux", "fuz", "bwx", "bwz")}
+        sn[name][:, :, slot] = 1.0
+        out[name] = O.e2drtm(iv, c, A, sn)
+    X = out["fux"][1] / 2
+    Z = out["bwz"][1] / 2
+    np.testing.assert_allclose(out["fux"][0], X + Z, rtol=1e-12, atol=1e-300)
+    return X, Z, out["fuz"][1]
+
+
+def _adj_stress_step22(c, XX, ZZ, XZ, UU, WW, dtx, pt):
+    """Adjoint stress update (e2drtm cpp:245-248) and free surface (cpp:270), order 22."""
+    t = c["temp"]
+    B = (slice(1, -1), slice(1, -1))
+    xx, zz, xz = XX.copy(), ZZ.copy(), XZ.copy()
+    xx[B] = t[B] * XX[B] - dtx * (UU[1:-1, 2:] - UU[1:-1, 1:-1])
+    zz[B] = t[B] * ZZ[B] - dtx * (WW[1:-1, 1:-1] - WW[:-2, 1:-1])
+    xz[B] = t[B] * XZ[B] - dtx * (UU[2:, 1:-1] - UU[1:-1, 1:-1]) - dtx * (WW[1:-1, 1:-1] - WW[1:-1, :-2])
+    zz[pt, :] = 0
+    return xx, zz, xz
+
+
+@pytest.mark.parametrize("gz", [4, 2])
+def test_adjoint_two_step_closed_form(gz):
+    """Two adjoint steps after one adjoint impulse (eigen_e2drtm_single.cpp:182-187, 238-248, 270).
+
+    Step nt-2 turns the impulse into stresses; step nt-3 forms uu_b / ww_b from
+    them with the coefficients inside the differences, hand-derived cell by
+    cell below, then the stresses read back through the imaging of slot nt-3.
+    gz = 2 is the production receiver row, one below the zeroed zz row pt = 1.
+    """
+    nzbc, nxbc, nz, nx, nbc, pt = 12, 14, 6, 8, 3, 1
+    c = _rand_coefs(nzbc, nxbc, 17)
+    nt, in_wf, dtx, dt, amp = 4, 1, 0.41, 0.013, 1.7
+    gx0, ng, g0 = 2, 10, 4
+    xc = gx0 + g0
+    A = np.zeros((nt, ng))
+    A[nt - 2, g0] = amp
+    iv = O.adj_iv(nt, nzbc, nxbc, dtx, ng, nbc, gz + 1, gx0 + 1, 1, 22, in_wf, nz, nx, dt)
+    X, Z, XZr = _adj_stress_readout(c, iv, A, nt, nt - 3, nz, nx)
+    k = lambda name, i, j: c[name][i, j]
+    p, q = dt * amp, dtx * dt * amp
+    # after step nt-2: the impulse in ww_b, and the stresses it makes
+    Z1 = np.zeros((nzbc, nxbc))
+    XZ1 = np.zeros_like(Z1)
+    X1 = np.zeros_like(Z1)
+    Z1[gz, xc], Z1[gz + 1, xc] = -q, q
+    XZ1[gz, xc], XZ1[gz, xc + 1] = -q, q
+    # step nt-3, velocity: uu_b = -b (sum of (coef * stress) differences), by hand
+    U2 = np.zeros_like(Z1)
+    W2 = np.zeros_like(Z1)
+    U2[gz, xc] = k("b", gz, xc) * q * (k("cl", gz, xc) + k("cm1", gz, xc))
+    U2[gz, xc + 1] = -k("b", gz, xc + 1) * q * (k("cl", gz, xc) + k("cm1", gz, xc + 1))
+    U2[gz + 1, xc] = -k("b", gz + 1, xc) * q * (k("cl", gz + 1, xc) + k("cm1", gz, xc))
+    U2[gz + 1, xc + 1] = k("b", gz + 1, xc + 1) * q * (k("cl", gz + 1, xc) + k("cm1", gz, xc + 1))
+    W2[gz - 1, xc] = k("b1", gz - 1, xc) * k("ca", gz, xc) * q
+    W2[gz, xc] = (k("temp", gz, xc) * p - k("b1", gz, xc) * q *
+                  (k("ca", gz, xc) + k("ca", gz + 1, xc) + k("cm1", gz, xc) + k("cm1", gz, xc + 1)))
+    W2[gz + 1, xc] = k("b1", gz + 1, xc) * k("ca", gz + 1, xc) * q
+    W2[gz, xc - 1] = k("b1", gz, xc - 1) * k("cm1", gz, xc) * q
+    W2[gz, xc + 1] = k("b1", gz, xc + 1) * k("cm1", gz, xc + 1) * q
+    xx, zz, xz = _adj_stress_step22(c, X1, Z1, XZ1, U2, W2, dtx, pt)
+    sl = (slice(pt + 1, pt + 1 + nz), slice(nbc, nbc + nx))
+    np.testing.assert_allclose(X, xx[sl], rtol=1e-12, atol=1e-20)
+    np.testing.assert_allclose(Z, zz[sl], rtol=1e-12, atol=1e-20)
+    np.testing.assert_allclose(XZr, xz[sl], rtol=1e-12, atol=1e-20)
+    assert np.count_nonzero(X) >= 4 and np.count_nonzero(XZr) >= 6
+
+
+def _scatter_velocity(c, XX, ZZ, XZ, UU, WW, S, pt):
+    """Adjoint velocity half-step (e2drtm cpp:182-187 for S = (1,), cpp:188-197
+    for S = (S41, S42)) as a scatter: each nonzero coefficient-weighted stress
+    adds +-S_q to the velocity cells whose differences read it."""
+    n0, n1 = ZZ.shape
+    au = np.zeros((n0, n1))
+    aw = np.zeros((n0, n1))
+    for qq, Sq in enumerate(S, start=1):
+        for (r, s) in zip(*np.nonzero(XX)):
+            v = c["ca"][r, s] * XX[r, s]            # uu_b: (ca xx)[i, j+q-1] - (ca xx)[i, j-q]
+            au[r, s - qq + 1] += Sq * v
+            au[r, s + qq] -= Sq * v
+            v = c["cl"][r, s] * XX[r, s]            # ww_b: (cl xx)[i+q, j] - (cl xx)[i-q+1, j]
+            aw[r - qq, s] += Sq * v
+            aw[r + qq - 1, s] -= Sq * v
+        for (r, s) in zip(*np.nonzero(ZZ)):
+            v = c["cl"][r, s] * ZZ[r, s]            # uu_b: (cl zz)[i, j+q-1] - (cl zz)[i, j-q]
+            au[r, s - qq + 1] += Sq * v
+            au[r, s + qq] -= Sq * v
+            v = c["ca"][r, s] * ZZ[r, s]            # ww_b: (ca zz)[i+q, j] - (ca zz)[i-q+1, j]
+            aw[r - qq, s] += Sq * v
+            aw[r + qq - 1, s] -= Sq * v
+        for (r, s) in zip(*np.nonzero(XZ)):
+            v = c["cm1"][r, s] * XZ[r, s]           # uu_b: (cm1 xz)[i+q-1, j] - (cm1 xz)[i-q, j]
+            au[r - qq + 1, s] += Sq * v
+            au[r + qq, s] -= Sq * v
+            aw[r, s - qq] += Sq * v                 # ww_b: (cm1 xz)[i, j+q] - (cm1 xz)[i, j-q+1]
+            aw[r, s + qq - 1] -= Sq * v
+    B = (slice(pt, n0 - pt), slice(pt, n1 - pt))
+    U = np.zeros((n0, n1))
+    W = np.zeros((n0, n1))
+    U[B] = c["temp"][B] * UU[B] - c["b"][B] * au[B]
+    W[B] = c["temp"][B] * WW[B] - c["b1"][B] * aw[B]
+    return U, W
+
+
+def _adj_stress_step(c, XX, ZZ, XZ, U, W, dtx, S, pt):
+    """Adjoint stress update on the block (e2drtm cpp:245-268) and zz_b row pt = 0 (cpp:270)."""
+    n0, n1 = XX.shape
+    r0, r1, c0, c1 = pt, n0 - pt, pt, n1 - pt
+    B = (slice(r0, r1), slice(c0, c1))
+    sh = lambda a, di, dj: a[r0 + di:r1 + di, c0 + dj:c1 + dj]
+    t = c["temp"]
+    dx_ = sum(Sq * (sh(U, 0, q) - sh(U, 0, 1 - q)) for q, Sq in enumerate(S, start=1))
+    dz_ = sum(Sq * (sh(W, q - 1, 0) - sh(W, -q, 0)) for q, Sq in enumerate(S, start=1))
+    du_ = sum(Sq * (sh(U, q, 0) - sh(U, 1 - q, 0)) for q, Sq in enumerate(S, start=1))
+    dw_ = sum(Sq * (sh(W, 0, q - 1) - sh(W, 0, -q)) for q, Sq in enumerate(S, start=1))
+    xx, zz, xz = XX.copy(), ZZ.copy(), XZ.copy()
+    xx[B] = t[B] * XX[B] - dtx * dx_
+    zz[B] = t[B] * ZZ[B] - dtx * dz_
+    xz[B] = t[B] * XZ[B] - dtx * du_ - dtx * dw_
+    zz[pt, :] = 0
+    return xx, zz, xz
+
+
+@pytest.mark.parametrize("order", [22, 24])
+def test_adjoint_three_steps_scatter_form(order):
+    """Three adjoint steps after one impulse, so that xx_b is nonzero when the
+    third velocity half-step reads it (the ca / cl weights of xx_b in uu_b and
+    ww_b); velocities in the scatter form, stresses read back through the
+    imaging of slot nt-4."""
+    pt = (order - 20) // 2
+    S = (1.0,) if order == 22 else (S41, S42)
+    nzbc, nxbc, nz, nx = 16, 18, 8, 10
+    nbc = (nxbc - nx) // 2
+    c = _rand_coefs(nzbc, nxbc, 31 + order)
+    nt, dtx, dt, amp = 5, 0.33, 0.017, 0.9
+    gz, gx0, ng, g0 = pt + 4, pt + 1, 12, 6
+    xc = gx0 + g0
+    A = np.zeros((nt, ng))
+    A[nt - 2, g0] = amp
+    iv = O.adj_iv(nt, nzbc, nxbc, dtx, ng, nbc, gz + 1, gx0 + 1, 1, order, 1, nz, nx, dt)
+    X, Z, XZr = _adj_stress_readout(c, iv, A, nt, nt - 4, nz, nx)
+    zero = np.zeros((nzbc, nxbc))
+    U, W = zero.copy(), zero.copy()
+    XX, ZZ, XZ = zero.copy(), zero.copy(), zero.copy()
+    for step in range(3):
+        U, W = _scatter_velocity(c, XX, ZZ, XZ, U, W, S, pt)
+        if step == 0:
+            W[gz, xc] += dt * amp
+        XX, ZZ, XZ = _adj_stress_step(c, XX, ZZ, XZ, U, W, dtx, S, pt)
+    sl = (slice(pt + 1, pt + 1 + nz), slice(nbc, nbc + nx))
+    np.testing.assert_allclose(X, XX[sl], rtol=1e-11, atol=1e-22)
+    np.testing.assert_allclose(Z, ZZ[sl], rtol=1e-11, atol=1e-22)
+    np.testing.assert_allclose(XZr, XZ[sl], rtol=1e-11, atol=1e-22)
+    assert np.count_nonzero(X) >= 12
+
+
+def test_adjoint_order24_closed_forms():
+    """Order 24 in the adjoint loop: one step by hand (stress stencil with S41/S42,
+    e2drtm cpp:249-254), then a second step whose velocity comes from the
+    scatter form of cpp:188-197."""
+    nzbc, nxbc, nz, nx, pt = 14, 16, 6, 8, 2
+    nbc = (nxbc - nx) // 2
+    c = _rand_coefs(nzbc, nxbc, 23)
+    dtx, dt, amp = 0.37, 0.011, 1.3
+    gz, gx0, ng, g0 = 5, 3, 10, 4
+    xc = gx0 + g0
+    p, q = dt * amp, dtx * dt * amp
+    # one step: nt = 3, slot nt-2 = 1
+    A = np.zeros((3, ng))
+    A[1, g0] = amp
+    iv = O.adj_iv(3, nzbc, nxbc, dtx, ng, nbc, gz + 1, gx0 + 1, 1, 24, 1, nz, nx, dt)
+    X, Z, XZr = _adj_stress_readout(c, iv, A, 3, 1, nz, nx)
+    Z1 = np.zeros((nzbc, nxbc))
+    XZ1 = np.zeros_like(Z1)
+    Z1[gz, xc], Z1[gz + 1, xc], Z1[gz - 1, xc], Z1[gz + 2, xc] = -S41 * q, S41 * q, -S42 * q, S42 * q
+    XZ1[gz, xc], XZ1[gz, xc + 1], XZ1[gz, xc - 1], XZ1[gz, xc + 2] = -S41 * q, S41 * q, -S42 * q, S42 * q
+    sl = (slice(pt + 1, pt + 1 + nz), slice(nbc, nbc + nx))
+    assert not np.any(X)
+    np.testing.assert_allclose(Z, Z1[sl], rtol=1e-13, atol=0)
+    np.testing.assert_allclose(XZr, XZ1[sl], rtol=1e-13, atol=0)
+    # two steps: nt = 4, slot nt-3 = 1
+    A = np.zeros((4, ng))
+    A[2, g0] = amp
+    iv = O.adj_iv(4, nzbc, nxbc, dtx, ng, nbc, gz + 1, gx0 + 1, 1, 24, 1, nz, nx, dt)
+    X, Z, XZr = _adj_stress_readout(c, iv, A, 4, 1, nz, nx)
+    W1 = np.zeros_like(Z1)
+    W1[gz, xc] = p
+    U2, W2 = _scatter_velocity(c, np.zeros_like(Z1), Z1, XZ1, np.zeros_like(Z1), W1, (S41, S42), pt)
+    t = c["temp"]
+    B = (slice(pt, nzbc - pt), slice(pt, nxbc - pt))
+    r0, r1, c0, c1 = pt, nzbc - pt, pt, nxbc - pt
+    sh = lambda a, di, dj: a[r0 + di:r1 + di, c0 + dj:c1 + dj]
+    xx, zz, xz = np.zeros_like(Z1), Z1.copy(), XZ1.copy()
+    xx[B] = -dtx * (S41 * (sh(U2, 0, 1) - sh(U2, 0, 0)) + S42 * (sh(U2, 0, 2) - sh(U2, 0, -1)))
+    zz[B] = t[B] * Z1[B] - dtx * (S41 * (sh(W2, 0, 0) - sh(W2, -1, 0)) + S42 * (sh(W2, 1, 0) - sh(W2, -2, 0)))
+    xz[B] = (t[B] * XZ1[B] - dtx * (S41 * (sh(U2, 1, 0) - sh(U2, 0, 0)) + S42 * (sh(U2, 2, 0) - sh(U2, -1, 0)))
+             - dtx * (S41 * (sh(W2, 0, 0) - sh(W2, 0, -1)) + S42 * (sh(W2, 0, 1) - sh(W2, 0, -2))))
+    zz[pt, :] = 0
+    np.testing.assert_allclose(X, xx[sl], rtol=1e-12, atol=1e-20)
+    np.testing.assert_allclose(Z, zz[sl], rtol=1e-12, atol=1e-20)
+    np.testing.assert_allclose(XZr, xz[sl], rtol=1e-12, atol=1e-20)
+    assert np.count_nonzero(X) >= 8
+
+
+def test_forward_order24_one_step_closed_form():
+    """Order 24 in the forward loop (eigen_staggerfd_single.cpp:229-235, 307-311):
+    the strains of the hard source after step 0 (snapshot slot 0) and the
+    step-1 velocities they make, by hand."""
+    nzbc, nxbc, nz, nx, pt = 16, 18, 8, 10, 2
+    nbc = (nxbc - nx) // 2
+    c = _rand_coefs(nzbc, nxbc, 29)
+    dtx = 0.29
+    sz, sx = 7, 8                              # 0-based source cell, away from the free-surface row
+    s = np.zeros(2)
+    s[0] = 1.0
+    ng, gx0 = nxbc - 2 * pt, pt
+    out = {}
+    for row in (sz, sz + 1):
+        iv = O.fwd_iv(2, nzbc, nxbc, dtx, ng, sz + 1, sx + 1, row + 1, gx0 + 1, 1, 5, 24, 1, nz, nx)
+        out[row] = O.staggerfd(iv, c, s)
+    su, sw, sn = out[sz]
+    # strains of slot 0 (interior origin (pt+1, nbc))
+    iz, ix = sz - pt - 1, sx - nbc
+    cw = {0: S41, 1: -S41, -1: S42, 2: -S42}  # bwx along x and bwz along z around the impulse
+    ebwx = np.zeros((nz, nx))
+    ebwz = np.zeros((nz, nx))
+    for d, v in cw.items():
+        ebwx[iz, ix + d] = v
+        ebwz[iz + d, ix] = v
+    np.testing.assert_allclose(sn["bwx"][:, :, 0], ebwx, rtol=1e-13, atol=0)
+    np.testing.assert_allclose(sn["bwz"][:, :, 0], ebwz, rtol=1e-13, atol=0)
+    assert not np.any(sn["fux"][:, :, 0]) and not np.any(sn["fuz"][:, :, 0])
+    # step 1: uu on the source row from xx = dtx cl bwz and xz = dtx cm1 bwx
+    k = lambda name, i, j: c[name][i, j]
+    eu = np.zeros(ng)
+    for d, v in cw.items():
+        j = sx + d
+        eu[j - gx0] = k("b", sz, j) * dtx * S41 * v * (k("cl", sz, sx) + k("cm1", sz, j))
+    np.testing.assert_allclose(su[1], eu, rtol=1e-12, atol=1e-300)
+    assert sw[1, sx - gx0] == 0.0              # overwritten by s[1] = 0
+    # ww one row below the source, from zz = dtx ca bwz in the source column
+    ew = k("b1", sz + 1, sx) * dtx * S41 * (S41 * k("ca", sz + 1, sx) - S42 * k("ca", sz + 2, sx)
+                                           - S42 * k("ca", sz, sx))
+    np.testing.assert_allclose(out[sz + 1][1][1, sx - gx0], ew, rtol=1e-12)
