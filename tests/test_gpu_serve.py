@@ -97,3 +97,74 @@ def test_paused_requests_resume_exactly(templates):
     assert st["completed"] == 6 and st["peak_active"] >= 2
     assert st["rebuilds"] > 6                # selections changed while requests were in flight
     _check_same(templates, trace, out)
+
+
+def test_served_request_vs_oracle():
+    """One served request (3 shots of a 30 x 120 model, nt 1500) against the
+    FP64 oracle's restatement of SWD.m:171-201.  The served result is bitwise
+    the request solved alone; where the FP32 picks equal the FP64 picks its
+    dk_vs is within 1e-4 of the oracle's, otherwise the request's chain after
+    the picks is judged from the oracle's picks (a pick one bin away changes
+    the gradient by construction)."""
+    from oracle import swd_oracle as O
+    nz, nx, dt, nt = 30, 120, 1e-4, 1500
+    vs = np.tile(np.linspace(220, 420, nz)[:, None], (1, nx))
+    vs_true = vs * (1 + 0.04 * P.checkerboard(nz, nx, 15, 1.0, 1.0))
+    q = dict(nz=nz, nx=nx, nbc=10, fd_order=22, nt=nt, in_wf=5, dx=0.5, dt=dt, sx=np.array([21, 61, 101]),
+             sz=np.array([1, 1, 1]), ng=120, gx0=1, dg=1, gz=1, df=0.5, fmin=5.0, fmax=80.0, vmin=100.0,
+             vmax=800.0, ds=0.5, offset=120, m=0, M=1, ini=10, w=0.0, beta=10.0, ns=120,
+             is_index=np.array([21, 61, 101]))
+    s = O.ricker(30, dt, nt)
+    gx = np.arange(1, 121, dtype=float)
+    gz = np.ones(120)
+    radon_t = (q["df"], dt, 701, q["vmin"], q["vmax"], q["fmin"], q["fmax"], nt, 120, q["ds"], q["offset"], 0, 1)
+    radon = dict(df=q["df"], fmin=q["fmin"], fmax=q["fmax"], vmin=q["vmin"], vmax=q["vmax"], ds=q["ds"],
+                 offset=q["offset"], m=0, M=1)
+    cr0_r, cr0_l, shots, tot = [], [], [], 0
+    for k, sx in enumerate(q["sx"]):
+        _, sw, _ = O.staggerfd_eigend(int(sx), 10, nt, dt / 0.5, 0.5, dt, int(sx), 1, gx, gz, s, 1.73 * vs_true,
+                                      vs_true, 1, 22, "w", 0, 5, snapshots=False)
+        cr0_r.append(O.pickFDC1(O.rt_side(sw, "r", int(sx), *radon_t)[0], q["ini"], 2, 4) + q["vmin"])
+        cr0_l.append(O.pickFDC1(O.rt_side(sw, "l", int(sx), *radon_t)[0], q["ini"], 2, 4) + q["vmin"])
+        o = O.swd_shot(1.73 * vs, vs, s, 10, nt, 0.5, dt, int(sx), 1, gx, gz, 5, 22, int(sx), 120, radon,
+                       cr0_r[-1], cr0_l[-1], q["beta"], q["ini"], 0.0)
+        shots.append(o)
+        tot = tot + np.array([o["cl_img"], o["cm_img"], o["illum"]])
+    cr0_r, cr0_l = np.array(cr0_r).T, np.array(cr0_l).T
+    dk_ref = O.assemble_dk_vs(vs, *tot)
+    srv = Server(hbm_bytes=4 << 30, max_active=4, nt_block=(5, 5))
+    try:
+        t = srv.submit(q, 1.73 * vs, vs, s, cr0_r, cr0_l)
+        r = srv.wait(t)
+    finally:
+        srv.close()
+    e = Engine(q)
+    try:
+        e.set_wavelet(s)
+        e.set_observed_picks(cr0_r, cr0_l)
+        e.set_model(1.73 * vs, vs)
+        g = e.gradient()
+        np.testing.assert_array_equal(r["dk_vs"], g["dk_vs"])
+        np.testing.assert_array_equal(r["res_r"], g["res_r"])
+        ocr_r = np.array([o["cr_r"] for o in shots]).T
+        ocr_l = np.array([o["cr_l"] for o in shots]).T
+        assert np.max(np.abs(g["cr_r"] - ocr_r)) <= 1 and np.max(np.abs(g["cr_l"] - ocr_l)) <= 1
+        lit = tot[2] > 1e-60
+        same = np.array_equal(g["cr_r"], ocr_r) and np.array_equal(g["cr_l"], ocr_l)
+        if same:
+            dk = r["dk_vs"]
+            np.testing.assert_allclose(r["res_r"], [o["res_r"] for o in shots], rtol=1e-9)
+        else:
+            e.run("forward")
+            e.run("dispersion")
+            e.set_picks(ocr_r, ocr_l)
+            e.run("adjoint_source")
+            e.run("adjoint")
+            e.run("assemble")
+            dk = e.read("gradient")[0]
+    finally:
+        e.close()
+    err = float(np.linalg.norm(dk[lit] - dk_ref[lit]) / np.linalg.norm(dk_ref[lit]))
+    print(f"served request: picks equal to FP64: {same}; dk_vs rel-L2 vs FP64 over lit cells {err:.2e}"
+          + ("" if same else " (from the oracle's picks)"))
+    assert np.isfinite(dk[lit]).all() and err < 1e-4
