@@ -135,6 +135,19 @@ int swd_lhdisppick(const float *ml, int np, int npair, double vmin, const double
                    double *res, double *cr_pre);
 
 /*
+ * swd_lhdisppick without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results (bitwise) and errors as swd_lhdisppick; every
+ * device buffer comes from a fixed 4 MiB array per GPU (held until the
+ * process ends), so np * npair * 4 + 2 * npair * 8 bytes plus 1 KiB must fit
+ * in 4 MiB, else SWD_ERR_ARG (701 x 1024 fits; 2048 x 1024 does not).  Calls
+ * run on the per-thread default stream; calls on one GPU are serialised.
+ */
+int swd_lhdisppick_noalloc(const float *ml, int np, int npair, double vmin, const double *cr_0, int method, int ini,
+                           double *res, double *cr_pre);
+
+/*
  * SoftMax adjoint source.  Replaces core/ADWDgrad_1.m:1
  *   a_data_res = ADWDgrad_1(nt, ng, ns, npair, is, w, m, M, SoftArgNorm, gor, gol, space_M, sfbR, sfbL)
  * The saveForBackward structs are passed as their fields: mm (np x npair

@@ -45,6 +45,7 @@ SIGNATURES = {
     "swd_radon": (_I, [_I, _P, _I, _I, _I, _D, _D, _I, _D, _D, _D, _D, _D, _I, _I, _I, _P, _P, _P, _P, _P]),
     "swd_radon_f32": (_I, [_I, _P, _I, _I, _I, _D, _D, _I, _D, _D, _D, _D, _D, _I, _I, _I, _P, _P, _P, _P, _P]),
     "swd_lhdisppick": (_I, [_P, _I, _I, _D, _P, _I, _I, _P, _P]),
+    "swd_lhdisppick_noalloc": (_I, [_P, _I, _I, _D, _P, _I, _I, _P, _P]),
     "swd_adwdgrad1": (_I, [_I, _I, _I, _I, _I, _D, _I, _I, _D, _P, _P, _P, _P, _I, _I, _P, _P, _I, _I, _I, _D,
                            _D, _D, _I, _I, _P]),
     "swd_engine_create": (_I, [_P, _P]),
