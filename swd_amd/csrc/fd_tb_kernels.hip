@@ -451,30 +451,36 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
     const bool flush = pf_img && ((itl[limg] / in_wf) % kImgFlush) == 0;
     const size_t iplane = (size_t)nzi * nxi;
     const gptr<const T> snb = snap + (pf_img ? (size_t)(itl[limg] / in_wf) * nsf * iplane : 0);
+    // Image rows are addressed as a uniform row base (scalar registers) plus
+    // this lane's byte offset, like the state rows.  Snapshot slots are
+    // [row][fux, bwz, fuz + bwx][x] (the time-blocked adjoint takes only that
+    // layout); the FP32 partials are interleaved [cell][3], one stream instead
+    // of three planes (production adjoint 139 -> 133 ms mean over seven
+    // processes); the double kernels keep their FP64 images as planes.
+    // this lane's element offset in a snapshot field row / image row
+    // (64-bit, so the three interleaved partials of a cell stay one
+    // dwordx3 access)
+    const size_t lane_c = (size_t)max(icol, 0);
+    const size_t lane_p = kPart ? 3 * lane_c : lane_c;
     auto img_load = [&](ImgRow<T>& q, const int i) {
         const int r = i - limg - 1;
         if (pf_img && r >= z0 && r < z1 && r > pt && r <= pt + nzi && icol >= 0) {
-            const size_t io = (size_t)(r - pt - 1) * nxi + icol;
-            if (nsf == 3) {   // [row][fux, bwz, fuz + bwx][x]
-                const gptr<const T> sn = snb + (size_t)(r - pt - 1) * 3 * nxi + icol;
-                q.fux = stream_ld(sn);
-                q.fwz = stream_ld(sn + nxi);
-                q.fsh = stream_ld(sn + 2 * nxi);
-            } else {          // [fux, fuz, bwx, bwz] planes
-                const gptr<const T> sn = snb + io;
-                q.fux = stream_ld(sn);
-                q.fwz = stream_ld(sn + 3 * iplane);
-                q.fsh = stream_ld(sn + iplane) + stream_ld(sn + 2 * iplane);
+            const int rr = r - pt - 1;
+            const gptr<const T> sr = snb + (size_t)rr * 3 * nxi + lane_c;
+            q.fux = stream_ld(sr);
+            q.fwz = stream_ld(sr + nxi);
+            q.fsh = stream_ld(sr + 2 * nxi);
+            if constexpr (kPart) {
+                const gptr<const T> pr = imgp + (size_t)rr * 3 * nxi + lane_p;
+                q.m0 = stream_ld(pr);
+                q.m1 = stream_ld(pr + 1);
+                q.m2 = stream_ld(pr + 2);
+            } else {
+                const gptr<const T> ir = imgp + (size_t)rr * nxi + lane_p;
+                q.m0 = stream_ld(ir);
+                q.m1 = stream_ld(ir + iplane);
+                q.m2 = stream_ld(ir + 2 * iplane);
             }
-            // FP32 partials are interleaved [cell][3]: one stream instead of
-            // three planes (production adjoint 139 -> 133 ms mean over seven
-            // processes, without the slow page-placement mode); the double
-            // kernels read their FP64 images as planes
-            const size_t ps = kPart ? 1 : iplane;
-            const gptr<const T> im = imgp + (kPart ? 3 * io : io);
-            q.m0 = stream_ld(im);
-            q.m1 = stream_ld(im + ps);
-            q.m2 = stream_ld(im + 2 * ps);
         }
     };
 
@@ -531,16 +537,9 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
                            icol >= 0) {
                     const size_t plane = (size_t)nzi * nxi;
                     const size_t io = (size_t)(r - pt - 1) * nxi + icol;
-                    const gptr<const T> sb = snap + (size_t)(itl[l] / in_wf) * nsf * plane;
-                    const gptr<const T> sn = sb + io;
-                    double Fux, Fwz, Fsh;
-                    if (nsf == 3) {
-                        const gptr<const T> sc = sb + (size_t)(r - pt - 1) * 3 * nxi + icol;
-                        Fux = stream_ld(sc); Fwz = stream_ld(sc + nxi); Fsh = stream_ld(sc + 2 * nxi);
-                    } else {
-                        Fux = stream_ld(sn); Fwz = stream_ld(sn + 3 * plane);
-                        Fsh = (double)(stream_ld(sn + plane) + stream_ld(sn + 2 * plane));
-                    }
+                    const gptr<const T> sb = snap + (size_t)(itl[l] / in_wf) * 3 * plane;
+                    const gptr<const T> sc = sb + (size_t)(r - pt - 1) * 3 * nxi + icol;
+                    const double Fux = stream_ld(sc), Fwz = stream_ld(sc + nxi), Fsh = stream_ld(sc + 2 * nxi);
                     double c0, c1, c2;
                     image_terms(Fux, Fwz, Fsh, oxx, ozz, oxz, c0, c1, c2);
                     const gptr<double> im = img + io;
@@ -575,43 +574,50 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
         }
         if constexpr (kLean) {
             const int r = i - limg - 1;
-            if (pf_img && r >= z0 && r < z1 && r > pt && r <= pt + nzi && icol >= 0) {
-                // prefetched snapshot and image values of this row
-                double c0, c1, c2;
-                image_terms(icur.fux, icur.fwz, icur.fsh, sxx, szz, sxz, c0, c1, c2);
-                const size_t io = (size_t)(r - pt - 1) * nxi + icol;
-                const gptr<double> im = img + io;
-                if constexpr (kPart) {
-                    const gptr<T> pp = imgp + 3 * io;
-                    if (flush) {
-                        // image_flush as atomic adds without return: the same sum
-                        // (one lane per cell and launch), and no wait on an FP64
-                        // load (24 us less per flush launch at the production shape)
-                        double* imd = (double*)im;
-                        atomicAdd(imd, (double)icur.m0 + c0);
-                        atomicAdd(imd + iplane, (double)icur.m1 + c1);
-                        atomicAdd(imd + 2 * iplane, (double)icur.m2 + c2);
-                        stream_st(pp, T(0));
-                        stream_st(pp + 1, T(0));
-                        stream_st(pp + 2, T(0));
+            if (pf_img && r >= z0 && r < z1 && r > pt && r <= pt + nzi) {
+                const int rr = r - pt - 1;
+                const gptr<double> imr = img + (size_t)rr * nxi + lane_c;
+                // new FP32 partials of this row (0 after a flush)
+                float n0 = 0.f, n1 = 0.f, n2 = 0.f;
+                if (icol >= 0) {
+                    // prefetched snapshot and image values of this row
+                    double c0, c1, c2;
+                    image_terms(icur.fux, icur.fwz, icur.fsh, sxx, szz, sxz, c0, c1, c2);
+                    double* imd = (double*)imr;
+                    if constexpr (kPart) {
+                        if (flush) {
+                            // image_flush as atomic adds without return: the same sum
+                            // (one lane per cell and launch), and no wait on an FP64
+                            // load (24 us less per flush launch at the production shape)
+                            atomicAdd(imd, (double)icur.m0 + c0);
+                            atomicAdd(imd + iplane, (double)icur.m1 + c1);
+                            atomicAdd(imd + 2 * iplane, (double)icur.m2 + c2);
+                        } else {
+                            // image_partial for the three images, with one test on the common path
+                            double v0 = (double)icur.m0 + c0, v1 = (double)icur.m1 + c1, v2 = (double)icur.m2 + c2;
+                            // a spill is the only access to that FP64 image in this launch (one
+                            // imaging level per launch here), so it goes as an atomic add without
+                            // return: no wait for it on the path
+                            if (image_tiny(v0)) { atomicAdd(imd, v0); v0 = 0.0; }
+                            if (image_tiny(v1)) { atomicAdd(imd + iplane, v1); v1 = 0.0; }
+                            if (image_tiny(v2)) { atomicAdd(imd + 2 * iplane, v2); v2 = 0.0; }
+                            n0 = (float)v0;
+                            n1 = (float)v1;
+                            n2 = (float)v2;
+                        }
                     } else {
-                        // image_partial for the three images, with one test on the common path
-                        double v0 = (double)icur.m0 + c0, v1 = (double)icur.m1 + c1, v2 = (double)icur.m2 + c2;
-                        // a spill is the only access to that FP64 image in this launch (one
-                        // imaging level per launch here), so it goes as an atomic add without
-                        // return: no wait for it on the path
-                        double* imd = (double*)im;
-                        if (image_tiny(v0)) { atomicAdd(imd, v0); v0 = 0.0; }
-                        if (image_tiny(v1)) { atomicAdd(imd + iplane, v1); v1 = 0.0; }
-                        if (image_tiny(v2)) { atomicAdd(imd + 2 * iplane, v2); v2 = 0.0; }
-                        stream_st(pp, (float)v0);
-                        stream_st(pp + 1, (float)v1);
-                        stream_st(pp + 2, (float)v2);
+                        stream_st(imr, (double)icur.m0 + c0);
+                        stream_st(imr + iplane, (double)icur.m1 + c1);
+                        stream_st(imr + 2 * iplane, (double)icur.m2 + c2);
                     }
-                } else {
-                    stream_st(im, (double)icur.m0 + c0);
-                    stream_st(im + iplane, (double)icur.m1 + c1);
-                    stream_st(im + 2 * iplane, (double)icur.m2 + c2);
+                }
+                if constexpr (kPart) {
+                    if (icol >= 0) {
+                        const gptr<T> pr = imgp + (size_t)rr * 3 * nxi + lane_p;
+                        stream_st(pr, (T)n0);
+                        stream_st(pr + 1, (T)n1);
+                        stream_st(pr + 2, (T)n2);
+                    }
                 }
             }
         }

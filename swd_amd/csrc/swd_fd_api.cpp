@@ -300,14 +300,23 @@ static int e2drtm_impl(const T* iv, const T* const coef_in[7], const T* adj, con
         SWD_HIP(hipStreamSynchronize(st.s));
     }
     {
-        std::vector<T> h((size_t)nrec * 4 * splane);
-        for (int k = 0; k < nrec; ++k)
-            for (int f = 0; f < 4; ++f) {
-                T* dst = h.data() + ((size_t)k * 4 + f) * splane;
-                const T* s4 = snap_in[f] + (size_t)k * splane;
-                for (int iz = 0; iz < d.nz; ++iz)
-                    for (int ix = 0; ix < d.nx; ++ix) dst[(size_t)iz * d.nx + ix] = s4[(size_t)ix * d.nz + iz];
+        // the engine's snapshot layout [slot][row][fux, bwz, fuz + bwx][x]: the
+        // imaging condition reads only these three (cpp:275-279), and every
+        // kernel forms fuz + bwx in T, so the sums are the same as from planes
+        std::vector<T> h((size_t)nrec * 3 * splane);
+        for (int k = 0; k < nrec; ++k) {
+            const size_t so = (size_t)k * splane;
+            const T *fux = snap_in[0] + so, *fuz = snap_in[1] + so, *bwx = snap_in[2] + so, *bwz = snap_in[3] + so;
+            for (int iz = 0; iz < d.nz; ++iz) {
+                T* row = h.data() + ((size_t)k * d.nz + iz) * 3 * d.nx;
+                for (int ix = 0; ix < d.nx; ++ix) {
+                    const size_t c = (size_t)ix * d.nz + iz;
+                    row[ix] = fux[c];
+                    row[d.nx + ix] = bwz[c];
+                    row[2 * d.nx + ix] = fuz[c] + bwx[c];
+                }
             }
+        }
         if ((rc = snaps.upload(h.data(), h.size() * sizeof(T), st.s))) return rc;
         SWD_HIP(hipStreamSynchronize(st.s));
     }
@@ -338,7 +347,7 @@ static int e2drtm_impl(const T* iv, const T* const coef_in[7], const T* adj, con
     J.imgp = (float*)imgp.p;
     J.nzbc = d.nzbc; J.nxbc = d.nxbc; J.pitch = P; J.nz = d.nz; J.nx = d.nx; J.nbc = d.nbc; J.pt = d.pt;
     J.gz = d.gz; J.gx = d.gx; J.dg = d.dg; J.ng = d.ng;
-    J.nt = d.nt; J.in_wf = d.in_wf; J.start = 0; J.nsf = 4;
+    J.nt = d.nt; J.in_wf = d.in_wf; J.start = 0; J.nsf = 3;
     J.dtx = (T)d.dtx; J.dt = (T)d.dt;
     if (fused) {
         std::vector<FdJob<T>> jobs{J};

@@ -120,6 +120,7 @@ int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int steps, 
     for (const auto& J : jobs) {
         if (adj && std::is_same<T, float>::value && !J.imgp) { set_error("float adjoint jobs need partial images"); return kErrArg; }
         if (J.stype != 0 && J.stype != 5) tb_ok = false;
+        if (adj && J.nsf != 3) tb_ok = false;   // the time-blocked adjoint reads the 3-field snapshots
         if (J.start != 0 || J.launch0 != 0) { set_error("run_march jobs must start together"); return kErrArg; }
     }
     const int NT = tb_ok ? nt_block : 1;
@@ -216,6 +217,7 @@ int run_march_graph(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int s
     for (const auto& J : jobs) {
         if (adj && std::is_same<T, float>::value && !J.imgp) { set_error("float adjoint jobs need partial images"); return kErrArg; }
         if (J.stype != 0 && J.stype != 5) tb_ok = false;
+        if (adj && J.nsf != 3) tb_ok = false;   // the time-blocked adjoint reads the 3-field snapshots
         if (J.start != 0 || J.launch0 != 0) { set_error("run_march jobs must start together"); return kErrArg; }
     }
     const int NT = tb_ok ? nt_block : 1;
