@@ -29,13 +29,15 @@ PEAK_HBM_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
 
 def fd_bytes_per_launch(prob, nshots, adjoint):
-    """Algorithmic HBM bytes of one FD time-step launch over the batch (FP32):
-    60 B per padded cell-step (5 state fields read + written, 5 coefficients
-    read) plus, averaged over the in_wf-step snapshot cycle, 16 B per interior
-    cell per snapshot step (forward: strain written; adjoint: strain read) and,
-    for the adjoint, the image update: an FP32 read-modify-write of the 3
-    partial images (24 B) plus, every 32nd imaging step, the FP64
-    read-modify-write of the 3 images (48 B / 32)."""
+    """Nominal HBM bytes of one FD time step over the batch (FP32), the
+    per-step model of an unblocked stencil: 60 B per padded cell-step (5 state
+    fields read + written, 5 coefficients read) plus, averaged over the
+    in_wf-step snapshot cycle, 16 B per interior cell per snapshot step
+    (forward: strain written; adjoint: strain read) and, for the adjoint, the
+    image update: an FP32 read-modify-write of the 3 partial images (24 B)
+    plus, every 32nd imaging step, the FP64 read-modify-write of the 3 images
+    (48 B / 32).  A time-blocked launch of k steps never moves the
+    intermediate steps' bytes, so k x this exceeds what it moves."""
     nzbc = prob["nz"] + 2 + prob["nbc"]
     nxbc = prob["nx"] + 2 * prob["nbc"]
     cells = nzbc * nxbc
@@ -46,13 +48,52 @@ def fd_bytes_per_launch(prob, nshots, adjoint):
     return per * nshots
 
 
-def cpu_baseline(prob, mod, seconds_hint=15.0):
-    """The FP32 statement-structure C port of the MEX time loops (oracle/),
-    one shot per thread on the host cores: forward + adjoint of `threads`
-    shots of the production grid with nt cut to keep the sample ~10-30 s."""
+def fd_compulsory_bytes_per_launch(prob, nshots, adjoint, k):
+    """Bytes a time-blocked launch of k steps must move to or from HBM, as the
+    kernels lay the data out: each shot's state (5 fields) read once and
+    written once (40 B per padded cell); the coefficient planes (temp, b, ca,
+    cl, cm1; b1 = b) read once per model (20 B per padded cell); per imaging
+    step (k / in_wf per launch on average) and interior cell the 3-field
+    snapshot (12 B: fux, bwz, fuz + bwx; written forward, read adjoint) and,
+    adjoint only, the FP32 partial-image read-modify-write (24 B) and the
+    FP64 flush every 32nd imaging step (3 x 8 B read + written, 48 B / 32);
+    receivers or adjoint-source samples (4 B per receiver and step)."""
+    nzbc = prob["nz"] + 2 + prob["nbc"]
+    nxbc = prob["nx"] + 2 * prob["nbc"]
+    cells = nzbc * nxbc
+    inner = prob["nz"] * prob["nx"]
+    imaging = k / prob["in_wf"]
+    per_shot = 40.0 * cells + imaging * 12.0 * inner + 4.0 * k * prob["ng"]
+    if adjoint:
+        per_shot += imaging * (24.0 + 48.0 / 32) * inner
+    return per_shot * nshots + 20.0 * cells
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(prob, mod, max_threads=16):
+    """The FP32 statement-structure C port of the MEX time loops (oracle/):
+    forward + adjoint of production-grid shots at the full nt, one shot per
+    thread (a parpool worker of the reference is single-threaded), on the
+    host cores this job may use, at most `max_threads`: the GPU box gives
+    one GPU's job a 16-core share whatever nproc reports.  Also times one
+    shot alone on one core."""
     from oracle import swd_oracle as O
     L = O.lib()
-    threads = max(1, min(16, os.cpu_count() or 1))
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    threads = max(1, min(max_threads, avail))
     nz, nx, nbc = prob["nz"], prob["nx"], prob["nbc"]
     nzbc, nxbc = nz + 2 + nbc, nx + 2 * nbc
     nt = prob["nt"]
@@ -64,23 +105,33 @@ def cpu_baseline(prob, mod, seconds_hint=15.0):
     ivb = np.array([nt, nzbc, nxbc, dtx, prob["ng"], nbc, 3, nbc + 1, 1, 22, prob["in_wf"], nz, nx, prob["dt"]],
                    dtype=np.float32)
     s = np.ascontiguousarray(mod["s"][:nt], dtype=np.float32)
-    sx = np.ascontiguousarray(prob["sx"][:threads] + nbc, dtype=np.int32)
-    nsh = sx.size
-    rng = np.random.default_rng(0)
-    adj = np.asfortranarray(rng.standard_normal((nsh * nt, prob["ng"])).astype(np.float32) * 1e-3)
-    sw = np.zeros(nsh * nt * prob["ng"], dtype=np.float32)
-    g = [np.zeros(nz * nx, dtype=np.float32) for _ in range(3)]
     p = O._p
-    t0 = time.perf_counter()
-    failed = L.oracle_batch_f(p(ivf), p(ivb), *[p(f32[k]) for k in O.COEF_ORDER], p(s), p(sx), nsh, p(adj), p(sw),
-                              p(g[0]), p(g[1]), p(g[2]), threads)
-    wall = time.perf_counter() - t0
-    if failed:
-        raise RuntimeError("CPU baseline shots failed")
-    cell_steps = nsh * nzbc * nxbc * (nt + nt - 1)
-    return {"value": cell_steps / wall / 1e9, "unit": "Gcell-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{nsh} shots (one per thread) of the 132x660 grid, nt={nt} of 6000, forward+adjoint, "
-                      f"FP32 C restatement of the MEX statement loops, {wall:.1f} s on {platform.processor() or 'host'}"}
+
+    def run(nsh, nthreads):
+        sx = np.ascontiguousarray((prob["sx"][:nsh] + nbc) % nxbc, dtype=np.int32)
+        rng = np.random.default_rng(0)
+        adj = np.asfortranarray(rng.standard_normal((nsh * nt, prob["ng"])).astype(np.float32) * 1e-3)
+        sw = np.zeros(nsh * nt * prob["ng"], dtype=np.float32)
+        g = [np.zeros(nz * nx, dtype=np.float32) for _ in range(3)]
+        t0 = time.perf_counter()
+        failed = L.oracle_batch_f(p(ivf), p(ivb), *[p(f32[k]) for k in O.COEF_ORDER], p(s), p(sx), nsh, p(adj),
+                                  p(sw), p(g[0]), p(g[1]), p(g[2]), nthreads)
+        wall = time.perf_counter() - t0
+        if failed:
+            raise RuntimeError("CPU baseline shots failed")
+        return wall
+
+    per_shot = nzbc * nxbc * (nt + nt - 1)
+    wall = run(threads, threads)
+    one = run(1, 1)
+    return {"value": round(threads * per_shot / wall / 1e9, 3), "unit": "Gcell-steps/s", "cores": threads,
+            "kind": "port",
+            "one_core_shot_s": round(one, 2), "one_core_gcell_steps_per_s": round(per_shot / one / 1e9, 3),
+            "nproc": os.cpu_count(), "affinity_cpus": avail, "cpu_model": _cpu_model(),
+            "sample": f"{threads} shots (one per thread, {threads} threads) of the 132x660 grid at the full nt={nt}, "
+                      f"forward + adjoint with imaging, FP32 C restatement of the MEX statement loops (one array pass "
+                      f"per Eigen statement), {wall:.1f} s; one shot alone on one core {one:.1f} s. FD only: the "
+                      f"Radon / pick / ADWDgrad_1 stages are not part of this CPU sample"}
 
 
 def copy_peak(nbytes=1 << 30, reps=10):
@@ -94,21 +145,26 @@ def copy_peak(nbytes=1 << 30, reps=10):
     return 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
 
-def measured_traffic(kname, shape, args):
-    """HBM bytes per launch of the kernel from the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE passes over the same workload (swd_amd/fd_traffic.json, written
-    by ``python -m swd_amd.profile_fd summarize``), when the measured launch
-    shape is the one that ran; else None."""
+N_SIMD = 1024           # MI355X: 256 CUs x 4 SIMDs
+CLOCK_HZ = 2.4e9        # MI355X peak engine clock
+
+
+def measured_profile(kname, shape, args):
+    """Per-launch HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a
+    copy kernel of known size) and VALU instruction count (SQ_INSTS_VALU) of
+    the kernel over the same workload, from swd_amd/fd_traffic.json (written
+    by ``python -m swd_amd.profile_fd summarize``), when the profiled launch
+    shape is the one that ran; else {}."""
     path = os.path.join(ROOT, "swd_amd", "fd_traffic.json")
     if not os.path.exists(path) or args.shots != 60 or args.nt != 6000:
-        return None
+        return {}
     with open(path) as fh:
         t = json.load(fh)
     k = kname.replace("_march", "_tb") if shape[0] > 1 else kname
     rec = t.get("kernels", {}).get(k)
     if rec is None or list(t.get("shapes", {}).get(k, [])) != list(shape):
-        return None
-    return int(rec["bytes_per_launch"])
+        return {}
+    return {"bytes_per_launch": int(rec["bytes_per_launch"]), "valu_insts": rec.get("valu_insts")}
 
 
 def progress(msg):
@@ -119,9 +175,8 @@ def uneven_leg(steps=2, graph=True):
     """The uneven multi-model batch: 8 faulted layered models of different grid
     size and nt (16 shots each) in shared FD launches, launch shape chosen by
     the autotuner on the batch.  Also times the same 8 gradients one model
-    after another, checks each model's gradient alone == in the mix, and gives
-    the gather error of one shot per model against the FP64 MEX path."""
-    from swd_amd import fd, problems as P
+    after another and checks each model's gradient alone == in the mix."""
+    from swd_amd import problems as P
     from swd_amd.engine import Engine, batch_autotune, batch_gradient
     models = P.uneven_models()
     engs = []
@@ -153,15 +208,6 @@ def uneven_leg(steps=2, graph=True):
     alone = [e.gradient()["dk_vs"] for e in engs]
     t_seq = time.perf_counter() - t0
     same = all(np.array_equal(a, b, equal_nan=True) for a, b in zip(alone, outs))
-    errs = []
-    for (prob, mod), e in zip(models, engs):
-        gx = prob["gx0"] + prob["dg"] * np.arange(prob["ng"])
-        nt = prob["nt"]
-        _, sw, _ = fd.staggerfd_eigend(1, prob["nbc"], nt, prob["dt"] / prob["dx"], prob["dx"], prob["dt"],
-                                      int(prob["sx"][0]), 1, gx, np.ones(prob["ng"]), mod["s"], mod["vp"], mod["vs"],
-                                      1, 22, "w", 0, nt, snapshots=False)
-        g32 = e.read("gathers")[0].astype(np.float64)
-        errs.append(float(np.linalg.norm(g32 - sw) / np.linalg.norm(sw)))
     for e in engs:
         e.close()
     return {"value": round(cell_steps / t_batch / 1e9, 3), "unit": "Gcell-steps/s",
@@ -171,9 +217,9 @@ def uneven_leg(steps=2, graph=True):
             "fd_launch": {"fwd": list(tune["fwd"]), "adj": list(tune["adj"]), "candidates": tune["candidates"],
                           "tune_s": round(tune_s, 2)},
             "alone_equals_in_mix": bool(same),
-            "gather_rel_l2_vs_fp64_max": max(errs),
-            "what": "8 models (nz x nx x nt x shots as listed), gradient of all 128 shots per step; error: FP32 "
-                    "batch gathers of shot 1 of each model vs the FP64 GPU MEX path"}
+            "what": "8 models (nz x nx x nt x shots as listed), gradient of all 128 shots per step; FP32 error "
+                    "against the FP64 oracle is a test (tests/test_gpu_configs.py::test_uneven_batch_full_size), "
+                    "not part of the timed run"}
 
 
 def iteration_leg(eng, mod, prob):
@@ -420,27 +466,49 @@ def main():
     ms_step = 1000.0 * wall / args.steps
     value = cell_steps / (wall / args.steps) / 1e9
 
-    # roofline of the dominant kernel (forward or adjoint march), event-timed in the run;
-    # one launch advances `k` time steps, so its algorithmic bytes are k steps' worth
+    # roofline of the dominant kernel (forward or adjoint march), its launch
+    # duration event-timed in the run; frac = compulsory bytes per launch (what
+    # a time-blocked launch must move, fd_compulsory_bytes_per_launch) over the
+    # duration, against the 8 TB/s HBM peak
     f_us, a_us = float(np.mean(fwd_us)), float(np.mean(adj_us))
     kf, ka = int(stage[-1]["fwd_steps_per_launch"]), int(stage[-1]["adj_steps_per_launch"])
     copy_gbs = copy_peak()
-    if f_us * nt / kf >= a_us * (nt - 1) / ka:
-        k = kf
-        kname, kus, kbytes = f"fd_fwd_{'tb' if k > 1 else 'march'}", f_us, k * fd_bytes_per_launch(prob, S, False)
-    else:
-        k = ka
-        kname, kus, kbytes = f"fd_adj_{'tb' if k > 1 else 'march'}", a_us, k * fd_bytes_per_launch(prob, S, True)
-    achieved = kbytes / (kus * 1e-6) / 1e9
-    kshape = [k, int(stage[-1]["fwd_band_rows" if kname.startswith("fd_fwd") else "adj_band_rows"])]
-    roof_traffic = measured_traffic(kname, kshape, args)
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": roof_traffic,
-            "kernel": kname, "launch_shape": kshape,
-            "hbm_gbs_measured": (round(roof_traffic / (kus * 1e-6) / 1e9, 1) if roof_traffic else None),
-            "copy_peak_gbs": round(copy_gbs, 1),
-            "hbm_frac_of_copy_peak": (round(roof_traffic / (kus * 1e-6) / 1e9 / copy_gbs, 3) if roof_traffic else None),
-            "kernel_us": round(kus, 2), "steps_per_launch": k, "algorithmic_bytes_per_launch": int(kbytes)}
+
+    def roofline(adjoint):
+        k, kus = (ka, a_us) if adjoint else (kf, f_us)
+        kname = f"fd_{'adj' if adjoint else 'fwd'}_{'tb' if k > 1 else 'march'}"
+        kshape = [k, int(stage[-1]["adj_band_rows" if adjoint else "fwd_band_rows"])]
+        comp = fd_compulsory_bytes_per_launch(prob, S, adjoint, k)
+        nominal = k * fd_bytes_per_launch(prob, S, adjoint)
+        prof = measured_profile(kname, kshape, args)
+        traffic = prof.get("bytes_per_launch")
+        achieved = comp / (kus * 1e-6) / 1e9
+        r = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+             "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+             "kernel": kname, "launch_shape": kshape, "kernel_us": round(kus, 2), "steps_per_launch": k,
+             "bytes_per_launch": int(comp),
+             "bytes_model": "compulsory bytes of one launch: state read + written once (40 B per padded cell and "
+                            "shot), coefficients once per model (20 B per padded cell), per imaging step 12 B of "
+                            "snapshot per interior cell and shot, adjoint also 24 B of FP32 partial-image RMW and "
+                            "48 B / 32 of FP64 flush, receivers 4 B per trace sample",
+             "traffic_over_compulsory": round(traffic / comp, 3) if traffic else None,
+             "hbm_gbs_measured": round(traffic / (kus * 1e-6) / 1e9, 1) if traffic else None,
+             "hbm_frac_measured": round(traffic / (kus * 1e-6) / 1e9 / PEAK_HBM_GBS, 4) if traffic else None,
+             "copy_peak_gbs": round(copy_gbs, 1),
+             "nominal_bytes_per_launch": int(nominal),
+             "nominal_gbs": round(nominal / (kus * 1e-6) / 1e9, 1),
+             "nominal_model": "k x the unblocked per-step model (60 B per padded cell-step + snapshot and image "
+                              "streams); exceeds the HBM peak because time blocking keeps the intermediate steps "
+                              "on chip"}
+        if prof.get("valu_insts"):
+            r["valu_busy_frac"] = round(prof["valu_insts"] * 4.0 / (N_SIMD * CLOCK_HZ * kus * 1e-6), 3)
+            r["valu_model"] = ("SQ_INSTS_VALU per launch x 4 cycles per wave64 instruction over 1024 SIMDs at the "
+                               "2.4 GHz peak engine clock, from the committed SQ counter pass")
+        return r
+
+    adj_dominant = a_us * (nt - 1) / ka > f_us * nt / kf
+    roof = roofline(adj_dominant)
+    roof_other = roofline(not adj_dominant)
 
     out = {"metric": "Gcell-steps/s FD fwd+adj (% HBM roofline), shots-grad/s; p50/p99 req latency",
            "value": round(value, 3), "unit": "Gcell-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -457,6 +525,7 @@ def main():
            "stage_ms": {k: round(float(np.mean([s[k] for s in stage])), 2)
                         for k in ("fwd_ms", "disp_ms", "adj_ms", "asm_ms")},
            "roofline": roof,
+           "roofline_other_phase": roof_other,
            "fd_launch": {"fwd": [kf, int(stage[-1]["fwd_band_rows"])], "adj": [ka, int(stage[-1]["adj_band_rows"])],
                          "autotune": tuned, "hip_graph": not args.no_graph},
            "device_bytes": info["device_bytes"],

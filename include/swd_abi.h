@@ -370,7 +370,7 @@ int swd_problem_npair(const swd_problem *p, int *npair);
  * admitted requests' shots advance together in shared FD launches, joining
  * and leaving the launch sequence independently, and each request's result
  * is bitwise the result of swd_engine_gradient on the same inputs.  Each
- * launch advances the requests with the least remaining work until their
+ * launch advances the requests with the earliest deadlines until their
  * shots fill the GPU once; the others wait with their wavefields resident.
  *
  * swd_server_create: hbm_bytes = device budget for request buffers (0: free
@@ -398,11 +398,20 @@ int swd_server_submit(swd_server *s, const swd_problem *p, const double *vp, con
 int swd_server_wait(swd_server *s, long long ticket, double timeout_ms, double *dk_vs, double *res_r, double *res_l,
                     double *times_ms);
 int swd_server_stats(swd_server *s, long long *vals, int n);
-/* Launch selection: requests are advanced least remaining work first until
- * their shots fill this many resident waves (0: the device's own count, the
- * default); the others wait with their wavefields resident, at most 150 ms
- * per FD phase. */
+/* Launch selection: requests are advanced earliest deadline first until their
+ * shots fill this many resident waves (0: the device's own count, the
+ * default); the others wait with their wavefields resident.  A request's
+ * deadline is its arrival time plus slack x its work (cell-steps) at a nominal
+ * 250 Gcell-steps/s, so new small requests come before large ones, and a large
+ * request overtakes the ones that arrive after it (slack > 0, default 2). */
 int swd_server_set_wave_slots(swd_server *s, long long waves);
+int swd_server_set_deadline_slack(swd_server *s, double slack);
+/* Cancel a request: a queued one ends at once, an admitted one leaves the
+ * launch sequence at the next launch boundary and its memory returns to the
+ * arenas once the launches already queued have drained; the other requests'
+ * results do not change.  swd_server_wait on it then returns SWD_ERR_BUSY.
+ * SWD_ERR_ARG for an unknown or finished ticket. */
+int swd_server_cancel(swd_server *s, long long ticket);
 
 #ifdef __cplusplus
 }

@@ -90,6 +90,8 @@ SIGNATURES = {
     "swd_server_wait": (_I, [_P, ctypes.c_longlong, ctypes.c_double, _P, _P, _P, _P]),
     "swd_server_stats": (_I, [_P, _P, _I]),
     "swd_server_set_wave_slots": (_I, [_P, ctypes.c_longlong]),
+    "swd_server_set_deadline_slack": (_I, [_P, _D]),
+    "swd_server_cancel": (_I, [_P, ctypes.c_longlong]),
 }
 
 _lock = threading.Lock()

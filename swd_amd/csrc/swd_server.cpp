@@ -21,6 +21,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -170,8 +171,8 @@ struct Request {
     bool sel = false;
     long long waves = 0;
     double work_per_launch = 0, work_after = 0;
-    double paused_since = -1;
-    bool pinned = false;  // waited kMaxPauseMs: stays selected until its phase ends
+    double deadline = 0;  // selection order: arrival + slack x the request's work at the nominal rate
+    std::atomic<bool> cancel{false};  // set by swd_server_cancel; the scheduler retires it
     std::vector<FdJob<float>> jobs;
     hipEvent_t ev_fd = nullptr, ev = nullptr;  // last FD launch of a phase; end of a side-stream stage
     hipStream_t side = nullptr;
@@ -188,7 +189,9 @@ constexpr int kMaxJobs = 1 << 14;       // shots in one FD launch table
 constexpr int kMaxBlocks = 1 << 20;     // workgroups of one launch
 constexpr int kStage = 4;               // pinned staging slots per FD phase
 constexpr int kMaxLaunchesPerEpoch = 32;
-constexpr double kMaxPauseMs = 150.0;   // a request paused this long is selected until its phase ends
+// nominal FD rate (cell-steps per ms) that turns a request's work into its
+// deadline: only the order of deadlines matters, so this just scales the slack
+constexpr double kNominalCellStepsPerMs = 2.5e8;
 
 }  // namespace
 
@@ -207,7 +210,10 @@ struct Server {
     std::condition_variable cv_work, cv_done, cv_admit;
     std::deque<Request*> queue;   // submitted, not yet admitted
     std::vector<Request*> ready;  // admitted by the admission thread, not yet seen by the scheduler
-    std::vector<Request*> active; // admitted, not finished (admission slots)
+    std::vector<Request*> active; // admitted, not finished (admission slots); entered with ready
+    Request* admitting = nullptr; // taken off the queue by the admission thread, being set up
+    double slack = 2.0;           // deadline = arrival + slack x work / nominal rate
+    long long fail_rebuild_at = -1;  // test hook: this launch-table rebuild fails (SWD_SERVER_FAIL_REBUILD)
     std::map<long long, std::unique_ptr<Request>> reqs;
     long long next_id = 1;
     bool stop = false;
@@ -248,6 +254,7 @@ struct Server {
     int end_phase(Request* r, int ph);
     void retire(Request* r);
     void fail_all(int rc, const char* what);
+    void drop_cancelled();
 };
 
 int Server::init(size_t hbm, int maxa, int ntf, int nta)
@@ -288,6 +295,7 @@ int Server::init(size_t hbm, int maxa, int ntf, int nta)
         }
         for (auto& e : epoch_ev[ph]) SWD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
+    if (const char* f = getenv("SWD_SERVER_FAIL_REBUILD")) fail_rebuild_at = atoll(f);
     th = std::thread([this] { loop(); });
     th_admit = std::thread([this] { admission_loop(); });
     return kOk;
@@ -355,12 +363,25 @@ void Server::fail(Request* r, int rc, const char* what)
     cv_admit.notify_all();
 }
 
-// Admission thread: in arrival order, while a slot is free, build the head
-// request's engine in the device arena and upload its model, wavelet and
-// observed picks; a request whose buffers do not fit waits until a running
-// request returns memory (or fails when nothing else holds any).
+// Admission thread: in arrival order, while a slot is free, take the head
+// request off the queue, build its engine in the device arena and upload its
+// model, wavelet and observed picks; a request whose buffers do not fit goes
+// back to the head of the queue until a running request returns memory (or
+// fails when nothing else holds any).  A request enters `active` (and
+// `ready`, for the scheduler) only once it is set up, so shutdown, failures of
+// the launch sequence and cancellation never see it half-built: until then it
+// is `admitting`, which only this thread touches.
 void Server::admission_loop()
 {
+    // fail the request being admitted; it is not in `active`, and once failed a
+    // waiter may free it, so it is not touched afterwards
+    auto give_up = [&](Request* r, int rc, const char* what) {
+        {
+            std::lock_guard<std::mutex> lk(m);
+            admitting = nullptr;
+        }
+        fail(r, rc, what);
+    };
     for (;;) {
         Request* r = nullptr;
         {
@@ -370,6 +391,8 @@ void Server::admission_loop()
             });
             if (stop) return;
             r = queue.front();
+            queue.pop_front();
+            admitting = r;
         }
         r->p.sx = r->sx.data();
         r->p.sz = r->sz.data();
@@ -385,48 +408,72 @@ void Server::admission_loop()
             e->st = nullptr;
             e.reset();
             std::unique_lock<std::mutex> lk(m);
-            if (!active.empty()) {
-                mem_blocked = true;  // cleared when a request returns its memory
+            if (!active.empty() && !stop && !r->cancel) {
+                queue.push_front(r);  // back at the head, until memory returns
+                admitting = nullptr;
+                mem_blocked = true;   // cleared when a request returns its memory
                 continue;
             }
-            queue.pop_front();
+            const bool quit = stop, cancelled = r->cancel;
             lk.unlock();
-            fail(r, rc, "request does not fit in the server's HBM budget");
+            if (quit) {
+                set_error("server shut down");
+                give_up(r, kErrArg, "server shut down before the request was admitted");
+            } else if (cancelled) {
+                set_error("cancelled by the caller");
+                give_up(r, kErrBusy, "request cancelled");
+            } else {
+                give_up(r, rc, "request does not fit in the server's HBM budget");
+            }
             continue;
-        }
-        {
-            std::lock_guard<std::mutex> lk(m);
-            queue.pop_front();
-            active.push_back(r);
-            peak_active = std::max<long long>(peak_active, (long long)active.size());
         }
         next_side = (next_side + 1) % (int)side.size();
         r->eng = std::move(e);
-        if (rc) { fail(r, rc, "engine create"); continue; }
+        if (rc) { give_up(r, rc, "engine create"); continue; }
         Engine& g = *r->eng;
         const size_t S = r->p.nshots;
         r->hbytes = g.splane * sizeof(double) + 2 * S * sizeof(double) + 2 * S * sizeof(int);
         r->hout = (double*)host.take(r->hbytes);
-        if (!r->hout) { set_error("pinned result arena exhausted"); fail(r, kErrNoMem, "admit"); continue; }
+        if (!r->hout) { set_error("pinned result arena exhausted"); give_up(r, kErrNoMem, "admit"); continue; }
         if ((rc = g.set_model(r->vp.data(), r->vs.data())) || (rc = g.set_wavelet(r->wav.data())) ||
             (rc = g.set_observed_picks(r->cr0r.data(), r->cr0l.data()))) {
-            fail(r, rc, "admit");
+            give_up(r, rc, "admit");
             continue;
         }
         if (hipEventCreateWithFlags(&r->ev_fd, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess) {
             (void)hipGetLastError();
             set_error("hipEventCreate failed");
-            fail(r, kErrHip, "admit");
+            give_up(r, kErrHip, "admit");
             continue;
         }
         r->jobs.clear();
         g.make_jobs(r->jobs, false, true);
+        const double work = (double)S * g.nzbc * g.nxbc * (2.0 * r->p.nt - 1);
+        bool quit = false, cancelled = false;
         {
             std::lock_guard<std::mutex> lk(m);
-            peak_bytes = std::max(peak_bytes, dev.used());
-            r->t_admit = now_ms();
-            ready.push_back(r);
+            quit = stop;
+            cancelled = r->cancel;
+            if (!quit && !cancelled) {
+                r->deadline = r->t_submit + slack * work / kNominalCellStepsPerMs;
+                active.push_back(r);
+                peak_active = std::max<long long>(peak_active, (long long)active.size());
+                peak_bytes = std::max(peak_bytes, dev.used());
+                r->t_admit = now_ms();
+                ready.push_back(r);
+                admitting = nullptr;
+            }
+        }
+        if (quit) {
+            set_error("server shut down");
+            give_up(r, kErrArg, "server shut down before the request was admitted");
+            continue;
+        }
+        if (cancelled) {
+            set_error("cancelled by the caller");
+            give_up(r, kErrBusy, "request cancelled");
+            continue;
         }
         cv_work.notify_all();
     }
@@ -441,7 +488,6 @@ static void begin_phase(Request* r, int ph, int NT)
     r->total = div_up(steps, NT);
     r->done = 0;
     r->sel = false;
-    r->pinned = false;
     const double cells = (double)r->p.nshots * g.nzbc * g.nxbc;
     r->work_per_launch = cells * NT;
     r->work_after = ph == 0 ? cells * (r->p.nt - 1) : 0.0;
@@ -456,9 +502,9 @@ void Server::progress()
         std::lock_guard<std::mutex> lk(m);
         act = active;
         for (Request* r : ready) {
+            if (r->phase == kFailed) continue;
             r->phase = kFwd;
             begin_phase(r, 0, nt_blk[0]);
-            r->paused_since = now_ms();
             members[0].push_back(r);
             reselect = true;
         }
@@ -478,7 +524,6 @@ void Server::progress()
             r->eng->make_jobs(r->jobs, true, true);
             r->phase = kAdj;
             begin_phase(r, 1, nt_blk[1]);
-            r->paused_since = now_ms();
             members[1].push_back(r);
             reselect = true;
         } else {
@@ -526,31 +571,30 @@ void Server::retire(Request* r)
     cv_admit.notify_all();
 }
 
-// Choose the requests that the next launches advance: least remaining work
-// first until the selected shots fill the GPU's resident waves once (then
-// every further request would only lengthen the launches), plus every
-// request that has waited kMaxPauseMs in its current phase (it then stays
-// selected until the phase ends, which bounds the wait of large requests).
+// Choose the requests that the next launches advance: earliest deadline first
+// (arrival time + slack x the request's work at a nominal rate) while their
+// shots fit in the GPU's resident waves (every further request would only
+// lengthen the launches).  A new small request comes before a
+// large one that arrived at about the same time, as with least remaining
+// work first, but a large request's deadline does not move, so it overtakes
+// the requests that keep arriving after it: waits stay bounded by the slack.
 void Server::select(double now)
 {
+    (void)now;
     reselect = false;
-    std::vector<std::pair<int, Request*>> c;
+    std::vector<Request*> c;
     for (int ph = 0; ph < 2; ++ph)
-        for (Request* r : members[ph]) c.push_back({ph, r});
-    std::stable_sort(c.begin(), c.end(), [](const std::pair<int, Request*>& a, const std::pair<int, Request*>& b) {
-        return a.second->remaining() < b.second->remaining();
-    });
+        for (Request* r : members[ph]) c.push_back(r);
+    std::stable_sort(c.begin(), c.end(), [](const Request* a, const Request* b) { return a->deadline < b->deadline; });
     long long waves = 0;
     const long long cap = slots_override.load() > 0 ? slots_override.load() : slots;
-    for (auto& pr : c) {
-        Request* r = pr.second;
-        if (r->paused_since >= 0 && now - r->paused_since > kMaxPauseMs) r->pinned = true;
-        const bool s = waves < cap || r->pinned;
-        if (s) waves += r->waves;
-        if (s != r->sel) {
-            r->sel = s;
-            r->paused_since = s ? -1.0 : now;
-        }
+    for (Request* r : c) {
+        // the earliest deadline always runs; the others join only where their
+        // waves still fit, so no launch holds more than the GPU does at once
+        // (a request that would overflow it waits instead of slowing every
+        // selected request down)
+        r->sel = waves == 0 || waves + r->waves <= cap;
+        if (r->sel) waves += r->waves;
     }
     for (int ph = 0; ph < 2; ++ph) {
         std::vector<Request*> t;
@@ -572,7 +616,10 @@ void Server::select(double now)
 // order, so launches already queued keep the table they were queued with.
 int Server::rebuild(int ph)
 {
-    ++n_rebuild;
+    if (++n_rebuild == fail_rebuild_at) {
+        set_error("launch-table rebuild %lld failed (SWD_SERVER_FAIL_REBUILD)", fail_rebuild_at);
+        return kErrArg;
+    }
     if (members[ph].empty()) gstep[ph] = glaunch[ph] = 0;
     if (gstep[ph] > (1 << 29)) { gstep[ph] = 0; glaunch[ph] = 0; }  // only differences matter
     const int NT = nt_blk[ph];
@@ -649,14 +696,47 @@ void Server::fail_all(int rc, const char* what)
     std::vector<Request*> act;
     {
         std::lock_guard<std::mutex> lk(m);
-        act = active;
+        act = active;   // every request of `ready` is in `active` too
+        ready.clear();
     }
-    for (Request* r : act) fail(r, rc, what);
     for (int ph = 0; ph < 2; ++ph) {
         members[ph].clear();
         table_of[ph].clear();
         nblocks[ph] = 0;
     }
+    for (Request* r : act) fail(r, rc, what);
+}
+
+// Retire cancelled requests: they leave the launch tables, the launches
+// already queued (which may still read their buffers) drain, and then their
+// memory goes back to the arenas.  The other requests continue from the same
+// state: their job origins are recomputed from the launches they were given.
+void Server::drop_cancelled()
+{
+    std::vector<Request*> cut;
+    {
+        std::lock_guard<std::mutex> lk(m);
+        for (Request* r : active)
+            if (r->cancel && r->phase != kFailed && r->phase != kDone) cut.push_back(r);
+        if (cut.empty()) return;
+        ready.erase(std::remove_if(ready.begin(), ready.end(), [](Request* r) { return r->cancel.load(); }),
+                    ready.end());
+    }
+    for (int ph = 0; ph < 2; ++ph) {
+        auto gone = [&](Request* r) { return std::find(cut.begin(), cut.end(), r) != cut.end(); };
+        members[ph].erase(std::remove_if(members[ph].begin(), members[ph].end(), gone), members[ph].end());
+        if (std::any_of(table_of[ph].begin(), table_of[ph].end(), gone)) {
+            table_of[ph].erase(std::remove_if(table_of[ph].begin(), table_of[ph].end(), gone), table_of[ph].end());
+            const int rc = rebuild(ph);
+            if (rc) { fail_all(rc, "FD launch table"); return; }
+        }
+    }
+    for (auto st : fd) (void)hipStreamSynchronize(st);
+    for (Request* r : cut) {
+        set_error("cancelled by the caller");
+        fail(r, kErrBusy, "request cancelled");
+    }
+    reselect = true;
 }
 
 void Server::loop()
@@ -670,13 +750,10 @@ void Server::loop()
                 continue;
             }
         }
+        drop_cancelled();
         progress();
         const double now = now_ms();
-        bool starved = false;
-        for (int ph = 0; ph < 2; ++ph)
-            for (Request* r : members[ph])
-                if (!r->sel && r->paused_since >= 0 && now - r->paused_since > kMaxPauseMs) starved = true;
-        if (reselect || starved) select(now);
+        if (reselect) select(now);
         for (int ph = 0; ph < 2; ++ph)
             if (gstep[ph] > (1 << 29)) {  // re-origin the table before the counters grow large
                 const int rc = rebuild(ph);
@@ -841,6 +918,43 @@ int swd_server_wait(swd_server* h, long long ticket, double timeout_ms, double* 
     }
     s.reqs.erase(it);
     return rc;
+}
+
+int swd_server_cancel(swd_server* h, long long ticket)
+{
+    if (!h) return kErrArg;
+    Server& s = h->s;
+    {
+        std::lock_guard<std::mutex> lk(s.m);
+        auto it = s.reqs.find(ticket);
+        if (it == s.reqs.end()) { set_error("unknown ticket %lld", ticket); return kErrArg; }
+        Request* r = it->second.get();
+        if (r->phase == kDone || r->phase == kFailed) { set_error("request %lld has finished", ticket); return kErrArg; }
+        auto q = std::find(s.queue.begin(), s.queue.end(), r);
+        if (q != s.queue.end()) {   // not admitted yet: it ends now
+            s.queue.erase(q);
+            r->rc = kErrBusy;
+            r->err = "request cancelled before admission";
+            r->t_done = s.now_ms();
+            ++s.n_failed;
+            r->phase = kFailed;
+        } else {
+            r->cancel = true;       // admitting or admitted: the server threads retire it
+        }
+    }
+    s.cv_done.notify_all();
+    s.cv_admit.notify_all();
+    s.cv_work.notify_all();
+    return kOk;
+}
+
+int swd_server_set_deadline_slack(swd_server* h, double slack)
+{
+    if (!h || !(slack > 0)) { set_error("deadline slack must be > 0"); return kErrArg; }
+    Server& s = h->s;
+    std::lock_guard<std::mutex> lk(s.m);
+    s.slack = slack;
+    return kOk;
 }
 
 int swd_server_set_wave_slots(swd_server* h, long long waves)

@@ -5,13 +5,15 @@ Run under rocprofv3 (one counter per pass):
     rocprofv3 --pmc FETCH_SIZE -d <dir>/fetch --output-format csv -- python -m swd_amd.profile_fd run
     rocprofv3 --pmc WRITE_SIZE -d <dir>/write --output-format csv -- python -m swd_amd.profile_fd run
     rocprofv3 --kernel-trace --stats -d <dir>/trace --output-format csv -- python -m swd_amd.profile_fd run
+    rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU -d <dir>/sq --output-format csv -- python -m swd_amd.profile_fd run
 
 ``run`` copies a known byte count with the calibration kernel, then computes
 one gradient of the production-shots workload (60 shots, 132 x 660 padded,
 nt 6000) with fixed forward / adjoint launch shapes.  ``summarize <dir>``
 reads the counter CSVs, scales the FD kernels' FETCH_SIZE / WRITE_SIZE by the
 copy kernel's (bytes known), and writes the per-launch HBM traffic of each
-time-blocked kernel to ``fd_traffic.json`` beside this file (read by the
+time-blocked kernel, and its VALU instruction count per launch when the
+``sq`` pass is there, to ``fd_traffic.json`` beside this file (read by the
 benchmark) and a markdown table to stdout.
 """
 from __future__ import annotations
@@ -114,6 +116,14 @@ def summarize(args):
         res["kernels"][s] = {"fetch_kib": fv, "write_kib": wv, "bytes_per_launch": b, "dispatches": n,
                              "name": k}
         lines.append(f"| {s} | {n} | {fv:.0f} | {wv:.0f} | {b / 1e6:.1f} MB |")
+    sq_dir = os.path.join(args.dir, "sq")
+    if os.path.isdir(sq_dir):
+        valu = _per_kernel(sq_dir, "SQ_INSTS_VALU")
+        for k, (v, n) in valu.items():
+            s = _short(k)
+            if s in res["kernels"] and res["kernels"][s]["name"] == k:
+                res["kernels"][s]["valu_insts"] = v
+                lines.append(f"| {s} | {n} | VALU instructions per dispatch {v / 1e6:.2f} M | | |")
     if args.shapes:
         res["shapes"] = json.loads(args.shapes)
     with open(TRAFFIC_JSON, "w") as fh:

@@ -5,7 +5,8 @@ A request is one forward + gradient solve of SWD.m:171-201 (all its shots)
 for one model.  The native server admits requests in arrival order while a
 slot is free and their buffers fit in the device budget, advances the shots
 of every admitted request in shared FD launches (requests join and leave the
-launch sequence independently) and runs each request's Radon / picking /
+launch sequence independently, earliest deadline first when they do not
+all fit the GPU at once) and runs each request's Radon / picking /
 adjoint-source and assembly stages on side streams.  ``max_active=1`` keeps
 one request on the GPU at a time; :func:`run_trace_per_request` is the plain
 per-request path (an Engine per request, one after another).
@@ -52,12 +53,21 @@ class Server:
         rr, rl, tm = np.zeros(S), np.zeros(S), np.zeros(4)
         rc = load().swd_server_wait(self.h, ticket, -1.0 if timeout is None else 1000.0 * timeout, ptr(dk), ptr(rr),
                                     ptr(rl), ptr(tm))
-        if rc == ERR_BUSY:
+        if rc == ERR_BUSY and b"still running" in (load().swd_last_error() or b""):
             raise TimeoutError(f"request {ticket} still running")
         del self._meta[ticket]
-        check(rc)
+        check(rc)   # a cancelled request raises SwdError (SWD_ERR_BUSY)
         return {"dk_vs": dk, "res_r": rr, "res_l": rl, "queued_ms": tm[0], "gpu_ms": tm[1], "total_ms": tm[2],
                 "submit_ms": tm[3]}
+
+    def cancel(self, ticket: int) -> None:
+        """Cancel a queued or running request (swd_server_cancel); its wait()
+        then raises SwdError with SWD_ERR_BUSY."""
+        check(load().swd_server_cancel(self.h, int(ticket)))
+
+    def set_deadline_slack(self, slack: float) -> None:
+        """Selection deadlines = arrival + slack x work at the nominal rate."""
+        check(load().swd_server_set_deadline_slack(self.h, float(slack)))
 
     def set_wave_slots(self, waves: int) -> None:
         """Resident waves the launch selection fills (0: the device's count)."""

@@ -168,3 +168,85 @@ def test_served_request_vs_oracle():
     print(f"served request: picks equal to FP64: {same}; dk_vs rel-L2 vs FP64 over lit cells {err:.2e}"
           + ("" if same else " (from the oracle's picks)"))
     assert np.isfinite(dk[lit]).all() and err < 1e-4
+
+
+def test_cancelled_requests_leave_the_others_unchanged(templates):
+    """A long request cancelled while on the GPU and one cancelled while
+    queued: both end with SWD_ERR_BUSY, every other result is bitwise its
+    result alone, and every buffer returns to the arena."""
+    import time
+    trace = poisson_trace(templates, rate=5000.0, n=6, seed=5, max_shots=6)
+    s = Server(hbm_bytes=8 << 30, max_active=2, nt_block=(4, 4))
+    q0, mod0, cr_r0, cr_l0 = templates[0]
+    big = dict(q0)
+    big["nt"] = q0["nt"] * 8      # runs long enough to be cancelled on the GPU
+    tickets = [s.submit(big, mod0["vp"], mod0["vs"], np.tile(mod0["s"], 8), cr_r0, cr_l0)]
+    for ta, ti, k in trace[1:]:
+        q, mod, cr_r, cr_l = templates[ti]
+        tickets.append(s.submit(subset(q, k), mod["vp"], mod["vs"], mod["s"], cr_r[:, :k], cr_l[:, :k]))
+    s.cancel(tickets[5])           # still queued behind two slots
+    time.sleep(0.03)
+    s.cancel(tickets[0])           # on the GPU
+    results = {}
+    for i, t in enumerate(tickets):
+        try:
+            results[i] = s.wait(t)
+        except SwdError as ex:
+            results[i] = ex
+    st = s.stats()
+    s.close()
+    assert isinstance(results[5], SwdError) and results[5].code == -8, results[5]
+    assert isinstance(results[0], SwdError) and results[0].code == -8, results[0]
+    assert st["bytes_in_use"] == 0 and st["active"] == 0
+    for i in range(1, 5):
+        r = results[i]
+        assert not isinstance(r, Exception), r
+        g, _ = _alone(templates[trace[i][1]], trace[i][2])
+        np.testing.assert_array_equal(r["dk_vs"], g["dk_vs"])
+        np.testing.assert_array_equal(r["res_r"], g["res_r"])
+
+
+def test_destroy_with_requests_queued_and_in_admission(templates):
+    """Shutting the server down while requests are queued, being admitted and
+    running: queued ones fail, the process carries on, a new server works."""
+    trace = poisson_trace(templates, rate=5000.0, n=8, seed=6, max_shots=6)
+    for delay in (0.0, 0.005, 0.05):
+        s = Server(hbm_bytes=8 << 30, max_active=2, nt_block=(4, 4))
+        for ta, ti, k in trace:
+            q, mod, cr_r, cr_l = templates[ti]
+            s.submit(subset(q, k), mod["vp"], mod["vs"], mod["s"], cr_r[:, :k], cr_l[:, :k])
+        if delay:
+            import time
+            time.sleep(delay)
+        s.close()
+    s = Server(hbm_bytes=8 << 30, max_active=2, nt_block=(4, 4))
+    out = run_trace(s, templates, trace[:2])
+    s.close()
+    _check_same(templates, trace[:2], out)
+
+
+def test_failed_launch_table_fails_the_admitted_requests_only(templates, monkeypatch):
+    """A launch-table rebuild that fails (test hook SWD_SERVER_FAIL_REBUILD)
+    fails the requests on the GPU and in the ready list, never one still being
+    admitted; later requests are served normally."""
+    monkeypatch.setenv("SWD_SERVER_FAIL_REBUILD", "3")
+    s = Server(hbm_bytes=8 << 30, max_active=3, nt_block=(4, 4))
+    monkeypatch.delenv("SWD_SERVER_FAIL_REBUILD")
+    trace = poisson_trace(templates, rate=5000.0, n=6, seed=7, max_shots=6)
+    tickets = []
+    for ta, ti, k in trace:
+        q, mod, cr_r, cr_l = templates[ti]
+        tickets.append(s.submit(subset(q, k), mod["vp"], mod["vs"], mod["s"], cr_r[:, :k], cr_l[:, :k]))
+    ok, failed = [], 0
+    for i, t in enumerate(tickets):
+        try:
+            ok.append((i, s.wait(t)))
+        except SwdError as ex:
+            assert ex.code == -2, ex
+            failed += 1
+    st = s.stats()
+    s.close()
+    assert failed >= 1 and st["bytes_in_use"] == 0 and st["active"] == 0
+    for i, r in ok:
+        g, _ = _alone(templates[trace[i][1]], trace[i][2])
+        np.testing.assert_array_equal(r["dk_vs"], g["dk_vs"])
