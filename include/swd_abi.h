@@ -133,6 +133,10 @@ int swd_radon_f32(int side, const float *seismo, int nt, int ng, int is, double 
  */
 int swd_lhdisppick(const float *ml, int np, int npair, double vmin, const double *cr_0, int method, int ini,
                    double *res, double *cr_pre);
+/* The same pick on FP64 spectra (SWD.m passes double ml to LHDispPick; the
+ * single entry above is SWD_single.m's). */
+int swd_lhdisppick_f64(const double *ml, int np, int npair, double vmin, const double *cr_0, int method, int ini,
+                       double *res, double *cr_pre);
 
 /*
  * swd_lhdisppick without device allocation.  No counterpart in the

@@ -45,6 +45,7 @@ SIGNATURES = {
     "swd_radon": (_I, [_I, _P, _I, _I, _I, _D, _D, _I, _D, _D, _D, _D, _D, _I, _I, _I, _P, _P, _P, _P, _P]),
     "swd_radon_f32": (_I, [_I, _P, _I, _I, _I, _D, _D, _I, _D, _D, _D, _D, _D, _I, _I, _I, _P, _P, _P, _P, _P]),
     "swd_lhdisppick": (_I, [_P, _I, _I, _D, _P, _I, _I, _P, _P]),
+    "swd_lhdisppick_f64": (_I, [_P, _I, _I, _D, _P, _I, _I, _P, _P]),
     "swd_lhdisppick_noalloc": (_I, [_P, _I, _I, _D, _P, _I, _I, _P, _P]),
     "swd_adwdgrad1": (_I, [_I, _I, _I, _I, _I, _D, _I, _I, _D, _P, _P, _P, _P, _I, _I, _P, _P, _I, _I, _I, _D,
                            _D, _D, _I, _I, _P]),
@@ -99,13 +100,15 @@ _lib = None
 
 
 def load() -> ctypes.CDLL:
-    """Load libswd.so (raises if it was not built)."""
+    """Load libswd.so (raises if it was not built).  SWD_LIBRARY names another
+    build of the same library instead (the sanitized one, libswd_san.so)."""
     global _lib
     with _lock:
         if _lib is None:
-            if not os.path.exists(LIB_PATH):
-                raise RuntimeError(f"{LIB_PATH} is missing: run `python -m swd_amd.build` first")
-            L = ctypes.CDLL(LIB_PATH)
+            path = os.environ.get("SWD_LIBRARY") or LIB_PATH
+            if not os.path.exists(path):
+                raise RuntimeError(f"{path} is missing: run `python -m swd_amd.build` first")
+            L = ctypes.CDLL(path)
             for name, (res, args) in SIGNATURES.items():
                 fn = getattr(L, name)
                 fn.restype = res

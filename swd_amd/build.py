@@ -66,5 +66,50 @@ def build(verbose: bool = True) -> str:
     return LIB
 
 
+SAN_LIB = os.path.join(HERE, "libswd_san.so")
+SAN_FLAGS = ["-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-fno-sanitize-recover=undefined"]
+
+
+def build_sanitized(verbose: bool = True) -> str:
+    """libswd_san.so: the host C++ sources (engine, server and arenas, MEX
+    argument marshalling, model preparation) compiled by g++ with
+    AddressSanitizer and UndefinedBehaviorSanitizer, linked with the HIP
+    objects of the normal build (device code is not instrumented).  Load it
+    with SWD_LIBRARY=<path> and LD_PRELOAD of gcc's libasan."""
+    os.makedirs(OBJ, exist_ok=True)
+    srcs = sources()
+    hip = [os.path.join(OBJ, f + ".o") for f in srcs if f.endswith(".hip")]
+    if not all(os.path.exists(o) for o in hip):
+        build(verbose=False)
+    gxx = shutil.which("g++") or "g++"
+
+    def comp(f):
+        obj = os.path.join(OBJ, f + ".san.o")
+        cmd = [gxx, "-std=c++17", "-O1", "-g", "-fPIC", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
+               "-I/opt/rocm/include", f"-I{CSRC}", f"-I{os.path.join(ROOT, 'include')}", "-c",
+               os.path.join(CSRC, f), "-o", obj] + SAN_FLAGS
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {f}:\n{' '.join(cmd)}\n{r.stderr}")
+        return obj
+
+    cpp = [f for f in srcs if f.endswith(".cpp")]
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(cpp))) as ex:
+        objs = list(ex.map(comp, cpp))
+    tmp = SAN_LIB + ".tmp"
+    cmd = [gxx, "-shared", "-o", tmp] + objs + hip + SAN_FLAGS + ["-L/opt/rocm/lib", "-lamdhip64",
+                                                                  "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stderr}")
+    os.replace(tmp, SAN_LIB)
+    if verbose:
+        print(f"built {SAN_LIB} (host code with ASan + UBSan)")
+    return SAN_LIB
+
+
 if __name__ == "__main__":
-    build()
+    if "--sanitize" in sys.argv:
+        build_sanitized()
+    else:
+        build()

@@ -690,28 +690,30 @@ __global__ __launch_bounds__(256) void radon_abs_kernel(const float2* MM, int np
 // of equal values whose previous sample is larger and whose run is followed
 // by a larger sample (MATLAB findpeaks defaults; endpoints never count).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float lds_or_nan(const float* v, int i, int n) { return (i >= 0 && i < n) ? v[i] : NAN; }
-
 // is position p (0-based within v[lo..hi]) a findpeaks location of -v ?
-__device__ bool is_local_min(const float* v, int lo, int hi, int p)
+template <typename V>
+__device__ bool is_local_min(const V* v, int lo, int hi, int p)
 {
     const int i = lo + p;
-    const float y = v[i];
+    const V y = v[i];
     if (y != y) return false;
     if (i == lo) return false;
-    const float yp = v[i - 1];
+    const V yp = v[i - 1];
     if (!(yp > y)) return false;  // previous (last of previous run) must be larger (NaN fails)
     int e = i;
     while (e + 1 <= hi && v[e + 1] == y) ++e;
     if (e + 1 > hi) return false;
-    const float yn = v[e + 1];
+    const V yn = v[e + 1];
     return yn > y;
 }
 
 // lp = 0 picks from the column of the spectrum's largest |mm| (the field
 // scripts' LHDispPick: [~,ini] = max(max(mlr))).  The misfit averages the
 // columns r0..npair-1-r1 (0-based; the field scripts' cr_pre(a:end-b)).
-__global__ __launch_bounds__(64) void pick_kernel(const float* ml, const float* mlr, const DispCol* cols, int np,
+// V is the spectrum's type: float for the engine and swd_lhdisppick, double
+// for callers that pick on FP64 spectra (swd_lhdisppick_f64).
+template <typename V>
+__global__ __launch_bounds__(64) void pick_kernel(const V* ml, const V* mlr, const DispCol* cols, int np,
                                                   int npair, int lp, float fg, float lfg, double vmin, int r0, int r1,
                                                   const double* cr0, double* cr_pre, double* res, int* status)
 {
@@ -723,17 +725,17 @@ __global__ __launch_bounds__(64) void pick_kernel(const float* ml, const float* 
     __shared__ double fidx[kMaxPair];
     // the frequency column being tracked and the next one, staged in LDS: the
     // next column's loads are in flight while the current one is searched
-    __shared__ float colv[2][kPickMaxNp];
+    __shared__ V colv[2][kPickMaxNp];
     if (!c.enabled) return;
-    const float* base = ml + (size_t)col * npair * np;
+    const V* base = ml + (size_t)col * npair * np;
     const int a = np;
     const int nk = (np + 63) >> 6;
-    float pf[kPickMaxNp / 64];
+    V pf[kPickMaxNp / 64];
     auto fetch = [&](int i) {
 #pragma unroll
         for (int k = 0; k < kPickMaxNp / 64; ++k) {
             const int j = k * 64 + lane;
-            if (k < nk) pf[k] = j < np ? base[(size_t)i * np + j] : 0.f;
+            if (k < nk) pf[k] = j < np ? base[(size_t)i * np + j] : V(0);
         }
     };
     auto put = [&](int b) {
@@ -745,22 +747,22 @@ __global__ __launch_bounds__(64) void pick_kernel(const float* ml, const float* 
     };
 
     // argmax over v[lo..hi] (NaN ignored, first index of the maximum); 1-based result
-    auto argmax = [&](const float* v, int lo, int hi) -> int {
-        float best = -INFINITY;
+    auto argmax = [&](const V* v, int lo, int hi) -> int {
+        V best = -INFINITY;
         int bi = 0x7fffffff;
         for (int i = lo + lane; i <= hi; i += 64) {
-            const float x = v[i];
+            const V x = v[i];
             if (x == x && (x > best || (x == best && i < bi))) { best = x; bi = i; }
         }
         for (int off = 32; off > 0; off >>= 1) {
-            const float ob = __shfl_xor(best, off);
+            const V ob = __shfl_xor(best, off);
             const int oi = __shfl_xor(bi, off);
             if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
         }
         return bi == 0x7fffffff ? 1 : bi - lo + 1;
     };
     // first local minimum at or after position 0 of v[lo..hi] (1-based location), 0 if none
-    auto first_min = [&](const float* v, int lo, int hi) -> int {
+    auto first_min = [&](const V* v, int lo, int hi) -> int {
         const int n = hi - lo + 1;
         for (int p0 = 0; p0 < n; p0 += 64) {
             const int p = p0 + lane;
@@ -771,7 +773,7 @@ __global__ __launch_bounds__(64) void pick_kernel(const float* ml, const float* 
         return 0;
     };
     // last local minimum of v[lo..hi] (1-based location), 0 if none
-    auto last_min = [&](const float* v, int lo, int hi) -> int {
+    auto last_min = [&](const V* v, int lo, int hi) -> int {
         const int n = hi - lo + 1;
         for (int p0 = ((n - 1) / 64) * 64; p0 >= 0; p0 -= 64) {
             const int p = p0 + lane;
@@ -784,13 +786,13 @@ __global__ __launch_bounds__(64) void pick_kernel(const float* ml, const float* 
 
     if (lp == 0) {
         // column maxima of |mm| (NaN ignored), then the first column holding the largest
-        const float* u = mlr + (size_t)col * npair * np;
-        float best = -INFINITY;
+        const V* u = mlr + (size_t)col * npair * np;
+        V best = -INFINITY;
         int bk = 0;
         for (int k = 0; k < npair; ++k) {
-            float m = -INFINITY;
-            for (int j = lane; j < np; j += 64) m = fmaxf(m, u[(size_t)k * np + j]);
-            for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+            V m = -INFINITY;
+            for (int j = lane; j < np; j += 64) m = fmax(m, u[(size_t)k * np + j]);
+            for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
             if (m > best) { best = m; bk = k; }
         }
         lp = bk + 1;
@@ -811,7 +813,7 @@ __global__ __launch_bounds__(64) void pick_kernel(const float* ml, const float* 
     int b = 1;
     for (int i = L1 - 1; i >= 0; --i) {
         if (i >= 1) fetch(i - 1);
-        const float* v = colv[b];
+        const V* v = colv[b];
         const int ip = idx[i + 1];
         if (ip == a) {
             if (lane == 0) { idx[i] = a; dvmax[i] = dvmax[i + 1]; dvmin[i] = dvmin[i + 1]; }
@@ -844,7 +846,7 @@ __global__ __launch_bounds__(64) void pick_kernel(const float* ml, const float* 
     b = 0;
     for (int i = L1 + 1; i < npair; ++i) {
         if (i + 1 < npair) fetch(i + 1);
-        const float* v = colv[b];
+        const V* v = colv[b];
         const int ip = idx[i - 1];
         int dmax, dmin;
         if (ip > a - 3) dmax = a;
@@ -898,19 +900,20 @@ __global__ __launch_bounds__(64) void pick_kernel(const float* ml, const float* 
 }
 
 // LHDispPick method 2 (LHDispPick.m:8-9): column argmax (first index, NaN ignored) + vmin.
-__global__ __launch_bounds__(64) void argmax_pick_kernel(const float* ml, int np, int npair, double vmin,
+template <typename V>
+__global__ __launch_bounds__(64) void argmax_pick_kernel(const V* ml, int np, int npair, double vmin,
                                                          const double* cr0, double* cr_pre)
 {
     const int k = blockIdx.x, lane = threadIdx.x;
-    const float* v = ml + (size_t)k * np;
-    float best = -INFINITY;
+    const V* v = ml + (size_t)k * np;
+    V best = -INFINITY;
     int bi = 0x7fffffff;
     for (int i = lane; i < np; i += 64) {
-        const float x = v[i];
+        const V x = v[i];
         if (x == x && (x > best || (x == best && i < bi))) { best = x; bi = i; }
     }
     for (int off = 32; off > 0; off >>= 1) {
-        const float ob = __shfl_xor(best, off);
+        const V ob = __shfl_xor(best, off);
         const int oi = __shfl_xor(bi, off);
         if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
     }
@@ -1087,20 +1090,44 @@ int launch_radon_abs(hipStream_t st, const float2* MM, int np, int npair, int nc
     return kOk;
 }
 
+template <typename V>
+static int pick_launch(hipStream_t st, const V* ml, const V* mlr, const DispCol* cols, int ncol, int np, int npair,
+                       int lp, float fg, float lfg, double vmin, int r0, int r1, const double* cr0, double* cr_pre,
+                       double* res, int* status)
+{
+    if (lp == 0 && !mlr) { set_error("an automatic pick start needs |mm|"); return kErrArg; }
+    pick_kernel<V><<<ncol, 64, 0, st>>>(ml, mlr, cols, np, npair, lp, fg, lfg, vmin, r0, r1, cr0, cr_pre, res, status);
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
 int launch_pick(hipStream_t st, const float* ml, const float* mlr, const DispCol* cols, int ncol, int np, int npair,
                 int lp, float fg, float lfg, double vmin, int r0, int r1, const double* cr0, double* cr_pre,
                 double* res, int* status)
 {
-    if (lp == 0 && !mlr) { set_error("an automatic pick start needs |mm|"); return kErrArg; }
-    pick_kernel<<<ncol, 64, 0, st>>>(ml, mlr, cols, np, npair, lp, fg, lfg, vmin, r0, r1, cr0, cr_pre, res, status);
-    SWD_HIP(hipGetLastError());
-    return kOk;
+    return pick_launch<float>(st, ml, mlr, cols, ncol, np, npair, lp, fg, lfg, vmin, r0, r1, cr0, cr_pre, res, status);
+}
+
+int launch_pick(hipStream_t st, const double* ml, const double* mlr, const DispCol* cols, int ncol, int np, int npair,
+                int lp, float fg, float lfg, double vmin, int r0, int r1, const double* cr0, double* cr_pre,
+                double* res, int* status)
+{
+    return pick_launch<double>(st, ml, mlr, cols, ncol, np, npair, lp, fg, lfg, vmin, r0, r1, cr0, cr_pre, res,
+                               status);
 }
 
 int launch_argmax_pick(hipStream_t st, const float* ml, int np, int npair, double vmin, const double* cr0,
                        double* cr_pre)
 {
-    argmax_pick_kernel<<<npair, 64, 0, st>>>(ml, np, npair, vmin, cr0, cr_pre);
+    argmax_pick_kernel<float><<<npair, 64, 0, st>>>(ml, np, npair, vmin, cr0, cr_pre);
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
+int launch_argmax_pick(hipStream_t st, const double* ml, int np, int npair, double vmin, const double* cr0,
+                       double* cr_pre)
+{
+    argmax_pick_kernel<double><<<npair, 64, 0, st>>>(ml, np, npair, vmin, cr0, cr_pre);
     SWD_HIP(hipGetLastError());
     return kOk;
 }

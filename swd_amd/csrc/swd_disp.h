@@ -71,6 +71,12 @@ int launch_pick(hipStream_t st, const float* ml, const float* mlr, const DispCol
                 double* res, int* status);
 int launch_argmax_pick(hipStream_t st, const float* ml, int np, int npair, double vmin, const double* cr0,
                        double* cr_pre);
+// the same on FP64 spectra
+int launch_pick(hipStream_t st, const double* ml, const double* mlr, const DispCol* cols, int ncol, int np, int npair,
+                int lp, float fg, float lfg, double vmin, int r0, int r1, const double* cr0, double* cr_pre,
+                double* res, int* status);
+int launch_argmax_pick(hipStream_t st, const double* ml, int np, int npair, double vmin, const double* cr0,
+                       double* cr_pre);
 int launch_softargmax(hipStream_t st, const float* mlr, const float2* MM, const DispCol* cols, int np, int npair,
                       int ncol, double beta, const double* cr_pre, const double* cr0, float2* GMM);
 int launch_adj_place(hipStream_t st, const float2* GD, const DispCol* cols, int ncol, int npair, int ng, int Lmax,

@@ -247,7 +247,8 @@ static int radon_impl(int side, const T* seismo, int nt, int ng, int is, double 
     return kOk;
 }
 
-static int lhdisppick_impl(const float* ml, int np, int npair, double vmin, const double* cr_0, int method, int ini,
+template <typename V>
+static int lhdisppick_impl(const V* ml, int np, int npair, double vmin, const double* cr_0, int method, int ini,
                            double* res, double* cr_pre)
 {
     if (!ml || !cr_0 || !cr_pre || np < 1 || npair < 1) { set_error("bad LHDispPick arguments"); return kErrArg; }
@@ -258,18 +259,18 @@ static int lhdisppick_impl(const float* ml, int np, int npair, double vmin, cons
     if ((rc = st.create())) return rc;
     DevBuf dml, dcr0, dpre, dres, dstat, dcols;
     DispCol c{1, 0, 0, 1, np};
-    if ((rc = dml.upload(ml, (size_t)np * npair * sizeof(float), st.s)) ||
+    if ((rc = dml.upload(ml, (size_t)np * npair * sizeof(V), st.s)) ||
         (rc = dcr0.upload(cr_0, (size_t)npair * sizeof(double), st.s)) || (rc = dpre.alloc(npair * sizeof(double))) ||
         (rc = dres.alloc(sizeof(double))) || (rc = dstat.alloc(sizeof(int))) ||
         (rc = dcols.upload(&c, sizeof c, st.s)))
         return rc;
     if (method == 1) {
         if (ini < 1 || ini > npair) { set_error("ini %d outside 1..%d", ini, npair); return kErrArg; }
-        if ((rc = launch_pick(st.s, dml.as<float>(), nullptr, dcols.as<DispCol>(), 1, np, npair, ini, 2.f, 4.f, vmin,
-                              0, 0, dcr0.as<double>(), dpre.as<double>(), dres.as<double>(), dstat.as<int>())))
+        if ((rc = launch_pick(st.s, dml.as<V>(), (const V*)nullptr, dcols.as<DispCol>(), 1, np, npair, ini, 2.f, 4.f,
+                              vmin, 0, 0, dcr0.as<double>(), dpre.as<double>(), dres.as<double>(), dstat.as<int>())))
             return rc;
     } else {
-        if ((rc = launch_argmax_pick(st.s, dml.as<float>(), np, npair, vmin, dcr0.as<double>(), dpre.as<double>())))
+        if ((rc = launch_argmax_pick(st.s, dml.as<V>(), np, npair, vmin, dcr0.as<double>(), dpre.as<double>())))
             return rc;
     }
     int stv = 0;
@@ -373,7 +374,13 @@ int swd_radon_f32(int side, const float* seismo, int nt, int ng, int is, double 
 int swd_lhdisppick(const float* ml, int np, int npair, double vmin, const double* cr_0, int method, int ini,
                    double* res, double* cr_pre)
 {
-    return lhdisppick_impl(ml, np, npair, vmin, cr_0, method, ini, res, cr_pre);
+    return lhdisppick_impl<float>(ml, np, npair, vmin, cr_0, method, ini, res, cr_pre);
+}
+
+int swd_lhdisppick_f64(const double* ml, int np, int npair, double vmin, const double* cr_0, int method, int ini,
+                       double* res, double* cr_pre)
+{
+    return lhdisppick_impl<double>(ml, np, npair, vmin, cr_0, method, ini, res, cr_pre);
 }
 
 int swd_adwdgrad1(int nt, int ng, int ns, int npair, int is, double w, int m, int M, double SoftArgNorm,

@@ -75,16 +75,18 @@ def RTlAD(seismo, is_, df, dt, np_, vmin, vmax, fmin, fmax, a, b, ds, offset, m,
 
 
 def LHDispPick(ml, vmin, cr_0, method, ini):
-    """[res, cr_pre] = LHDispPick(ml, vmin, cr_0, method, ini)."""
-    ml = fortran(ml, np.float32)
+    """[res, cr_pre] = LHDispPick(ml, vmin, cr_0, method, ini); FP64 spectra
+    are picked in FP64, others in FP32."""
+    f64 = np.asarray(ml).dtype == np.float64
+    ml = fortran(ml, np.float64 if f64 else np.float32)
     np_, npair = ml.shape
     cr_0 = np.ascontiguousarray(np.ravel(cr_0), dtype=np.float64)
     if cr_0.size != npair:
         raise ValueError("cr_0 must have one entry per frequency column")
     cr = np.zeros(npair)
     res = ctypes.c_double(0)
-    check(load().swd_lhdisppick(ptr(ml), np_, npair, float(vmin), ptr(cr_0), int(method), int(ini),
-                                ctypes.byref(res), ptr(cr)))
+    fn = load().swd_lhdisppick_f64 if f64 else load().swd_lhdisppick
+    check(fn(ptr(ml), np_, npair, float(vmin), ptr(cr_0), int(method), int(ini), ctypes.byref(res), ptr(cr)))
     return res.value, cr
 
 

@@ -21,8 +21,10 @@ void swd_staggerfd_mex(matlab::mex::Function& self, matlab::mex::ArgumentList ou
     for (size_t k = 1; k <= 8; ++k) a.push_back(take<T>(inputs, k));
     auto su = f.createBuffer<T>(nt * ng), sw = f.createBuffer<T>(nt * ng);
     const bool want_s = outputs.size() > 2;
-    matlab::data::buffer_ptr_t<T> sn[4];
-    for (auto& b : sn) b = f.createBuffer<T>(want_s ? nsn : 1);
+    // buffer_ptr_t has a function-pointer deleter: no default construction
+    const size_t nb = want_s ? nsn : 1;
+    matlab::data::buffer_ptr_t<T> sn[4] = {f.createBuffer<T>(nb), f.createBuffer<T>(nb), f.createBuffer<T>(nb),
+                                           f.createBuffer<T>(nb)};
     const int rc = call(v, data(a[0]), data(a[1]), data(a[2]), data(a[3]), data(a[4]), data(a[5]), data(a[6]),
                         data(a[7]), su.get(), sw.get(), want_s ? sn[0].get() : nullptr,
                         want_s ? sn[1].get() : nullptr, want_s ? sn[2].get() : nullptr,

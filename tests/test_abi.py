@@ -54,3 +54,24 @@ def test_mex_entry_rejects_bad_descriptors_before_touching_the_device():
         with pytest.raises(_lib.SwdError) as e:
             fd.eigen_staggerfd_single(iv, *[c[k] for k in O.COEF_ORDER], np.zeros(8))
         assert e.value.code == -2
+
+
+def test_adwdgrad1_rejects_ccn_below_nt_before_touching_the_device():
+    """ADWDgrad_1.m:42-43 reads graduxt(:, 1:nt) of an ifft of length ccn:
+    with ccn < nt MATLAB errors; the library returns SWD_ERR_ARG."""
+    import ctypes
+    L = _lib.load()
+    nt, ng, np_, npair = 600, 20, 50, 10
+    mm = np.ones((2 * np_, npair), dtype=np.float32, order="F")
+    ml = np.ones((np_, npair), dtype=np.float32, order="F")
+    g = np.zeros(npair)
+    out = np.zeros((nt, ng), dtype=np.float32, order="F")
+    P = _lib.ptr
+    rc = L.swd_adwdgrad1(nt, ng, 20, npair, 5, 0.0, 0, 1, 10.0, P(g), P(g), P(mm), P(ml), 5, 10, P(mm), P(ml), 5, 5,
+                         np_, 100.0, 0.5, 0.5, 11, 500, P(out))
+    assert rc == -2
+    assert b"ccn" in L.swd_last_error()
+    # the oracle's restatement fails the same way
+    sfb = {"lf": 11, "nf": 20, "ccn": 500}
+    with pytest.raises(ValueError):
+        O.adjoint_side(sfb, g, 10.0, nt)

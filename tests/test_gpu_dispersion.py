@@ -113,3 +113,19 @@ def test_adwdgrad_vs_oracle(gather, beta):
     e = rel(a, ref)
     print(f"ADWDgrad_1 beta={beta} fp32 rel-L2 vs fp64: {e:.3e}")
     assert e < 1e-4
+
+
+@pytest.mark.parametrize("side", ["r", "l"])
+def test_fp64_spectra_are_picked_in_fp64(gather, side):
+    """swd_lhdisppick_f64 (SWD.m passes double spectra): on the oracle's own
+    FP64 spectrum the picks are those of the oracle's pickFDC1, both methods,
+    with no rounding of the spectrum to FP32 in between."""
+    sw, radon, nt = gather
+    ml_o, _, _ = O.rt_side(sw, side, *radon)
+    assert ml_o.dtype == np.float64
+    cr0 = np.full(ml_o.shape[1], 300.0)
+    for method in (1, 2):
+        rg, cg = Dp.LHDispPick(ml_o, 100, cr0, method, 10)
+        ro, co = O.LHDispPick(ml_o, 100, cr0, method, 10)
+        np.testing.assert_array_equal(cg, co)
+        assert rg == pytest.approx(ro, rel=1e-15)
