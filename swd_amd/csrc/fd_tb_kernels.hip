@@ -95,6 +95,31 @@ __device__ __forceinline__ int xcd_block(int i, int n)
     return s * S + (k & 7) * G + (k >> 3);
 }
 
+// Issue priority from a wave's progress through its band.  The waves sharing
+// a SIMD start together with equal work, and the arbiter issues oldest first,
+// so without this they finish staggered (production forward: 46, 50, 56 and
+// 63 us) and the last one runs alone, its memory latency no longer hidden by
+// the others.  A wave drops priority as it passes 60, 80 and 93% of its rows,
+// so the ones behind catch up and the SIMD's waves finish within the last
+// stretch of rows.
+struct ProgressPrio {
+    int t1, t2, t3;
+    __device__ __forceinline__ ProgressPrio(int i0, int i1)
+    {
+        const int n = i1 - i0 + 1;
+        t1 = i0 + n * 60 / 100;
+        t2 = i0 + n * 80 / 100;
+        t3 = i0 + n * 93 / 100;
+    }
+    __device__ __forceinline__ void at(int i) const
+    {
+        if (i < t1) __builtin_amdgcn_s_setprio(3);
+        else if (i < t2) __builtin_amdgcn_s_setprio(2);
+        else if (i < t3) __builtin_amdgcn_s_setprio(1);
+        else __builtin_amdgcn_s_setprio(0);
+    }
+};
+
 // Minimum waves per SIMD the float kernels are compiled for (1 for the
 // double kernels: no constraint).  Float kernels at NT <= 5 fit 128 VGPRs
 // anyway; the hint changes the adjoint's instruction schedule (production
@@ -359,8 +384,10 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
     // Rows are loaded one step ahead into two alternating buffers, so the
     // loads of row i+1 are in flight while row i is processed.
     RowLoad<T, B1> A, B;
+    const ProgressPrio prio(i0, i1);
     A.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i0, nzbc, rowb, xb);
     for (int i = i0; i <= i1; i += 2) {
+        prio.at(i);
         B.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i + 1, nzbc, rowb, xb);
         row(A, i);
         if (i + 1 > i1) break;
@@ -637,9 +664,11 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
         constexpr bool kLean = decltype(lean)::value;
         RowLoad<T, B1> A, B;
         ImgRow<T> IA, IB;
+        const ProgressPrio prio(i0, i1);
         A.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i0, nzbc, rowb, xb);
         if constexpr (kLean) img_load(IA, i0);
         for (int i = i0; i <= i1; i += 2) {
+            prio.at(i);
             B.load(xx0, zz0, xz0, uu0, ww0, ctemp, cb, cb1, cca, ccl, ccm1, i + 1, nzbc, rowb, xb);
             if constexpr (kLean) img_load(IB, i + 1);
             row(A, IA, i, lean);
