@@ -78,23 +78,6 @@ __device__ __forceinline__ double tb_right(double v)
 // a step index clamped into [0, nt) (a level past nt reads a valid source sample it does not use)
 __device__ __forceinline__ int itl_clamp(int itl, int nt) { return min(itl, nt - 1); }
 
-// XCD-grouped block order.  The dispatcher places workgroup i on XCD i mod 8,
-// and each XCD has its own L2.  Within every group of 8 G workgroups, XCD x
-// gets G consecutive entries of the block list, so neighbouring tiles of a
-// shot, which share halo rows and columns, mostly share one L2; a trailing
-// partial group keeps the identity order.  A bijection on [0, n).  G = 8: the
-// production adjoint ran 10% faster than with the round-robin order, and an
-// uneven multi-model batch (jobs ending at different launches) as fast.
-// Giving each XCD one contiguous eighth of the list instead leaves XCDs whose
-// jobs have ended idle: that batch ran 30% slower.
-__device__ __forceinline__ int xcd_block(int i, int n)
-{
-    constexpr int G = 8, S = 8 * G;
-    if (i >= n / S * S) return i;
-    const int s = i / S, k = i - s * S;
-    return s * S + (k & 7) * G + (k >> 3);
-}
-
 // Issue priority from a wave's progress through its band.  The waves sharing
 // a SIMD start together with equal work, and the arbiter issues oldest first,
 // so without this they finish staggered (production forward: 46, 50, 56 and
@@ -237,7 +220,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
                                                                       int launch)
 {
     constexpr int W = kMarchLanes - 2 * NT;  // output columns per wavefront
-    const BlockMap bm = bmap[xcd_block(blockIdx.x, gridDim.x)];
+    const BlockMap bm = bmap[blockIdx.x];  // the list is in dispatch order
     const FdJob<T>& J = jobs[bm.job];
     const int it0 = step - J.start;
     if (it0 < 0 || it0 >= J.nt) return;
@@ -404,7 +387,7 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) __attribute__((amdgpu_wav
                                                                       int launch)
 {
     constexpr int W = kMarchLanes - 2 * NT;
-    const BlockMap bm = bmap[xcd_block(blockIdx.x, gridDim.x)];
+    const BlockMap bm = bmap[blockIdx.x];  // the list is in dispatch order
     const FdJob<T>& J = jobs[bm.job];
     const int k0 = step - J.start;
     if (k0 < 0 || J.nt - 2 - k0 < 0) return;

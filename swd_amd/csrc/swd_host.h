@@ -187,8 +187,12 @@ struct MarchPlan {
 
 // Tile every job of a batch for a march launch: out_cols output columns per
 // wavefront (62 single-step, 64 - 2 NT time-blocked) and tz rows per band.
+// The block list is in dispatch order (entry i runs as workgroup i); for a
+// time-blocked plan (NT > 1, adj: the adjoint kernel) it is arranged for
+// the XCDs' L2s and the CUs' load balance.
 template <typename T>
-int build_march_plan(std::vector<FdJob<T>>& jobs, int tz, MarchPlan& plan, int out_cols = kMarchOut);
+int build_march_plan(std::vector<FdJob<T>>& jobs, int tz, MarchPlan& plan, int out_cols = kMarchOut, int NT = 1,
+                     bool adj = false);
 
 template <typename T>
 int launch_fwd_march(hipStream_t st, const FdJob<T>* d_jobs, const BlockMap* d_bmap, int nblocks, int step,

@@ -99,4 +99,22 @@ constexpr int kMarchWaves = 4;           // wavefronts per workgroup
 
 inline int div_up(int a, int b) { return (a + b - 1) / b; }
 
+// XCD-grouped block order of the time-blocked launches: workgroup i (placed
+// on XCD i mod 8 by the dispatcher, each XCD with its own L2) runs entry
+// xcd_block(i, n) of the job-major block list.  Within every group of 8 G workgroups,
+// XCD x gets G consecutive entries, so neighbouring tiles of a shot, which
+// share halo rows and columns, mostly share one L2; a trailing partial group
+// keeps the identity order.  A bijection on [0, n).  G = 8: the production
+// adjoint ran 10% faster than with the round-robin order, and an uneven
+// multi-model batch (jobs ending at different launches) as fast.  Giving
+// each XCD one contiguous eighth of the list instead leaves XCDs whose jobs
+// have ended idle: that batch ran 30% slower.
+__host__ __device__ inline int xcd_block(int i, int n)
+{
+    constexpr int G = 8, S = 8 * G;
+    if (i >= n / S * S) return i;
+    const int s = i / S, k = i - s * S;
+    return s * S + (k & 7) * G + (k >> 3);
+}
+
 }  // namespace swd

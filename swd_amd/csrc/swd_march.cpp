@@ -23,24 +23,103 @@ void set_fd_time_block(int nt_block, int rows)
     g_tb_rows.store(std::max(rows, 0));
 }
 
+// Estimated work of one time-blocked tile, in rows of the level chain: the
+// rows the wavefront marches, plus its imaging (adjoint) or snapshot
+// (forward) rows weighted by their cost relative to a chain row and by the
+// share of the wavefront's columns they cover.  Weights from per-wave
+// timestamps of the production workload: with one imaging level per launch,
+// adjoint CUs holding only imaging tiles took 1.65 times as long as CUs
+// holding only absorbing-border tiles; snapshot stores cost the forward
+// about a tenth of that.
 template <typename T>
-int build_march_plan(std::vector<FdJob<T>>& jobs, int tz, MarchPlan& plan, int out_cols)
+static double tb_tile_work(const FdJob<T>& J, int tile, int out_cols, int NT, bool adj)
 {
-    plan.bmap.clear();
+    if (tile >= J.tiles_x * J.tiles_z) return 0.0;
+    const int tix = tile % J.tiles_x, tiz = tile / J.tiles_x;
+    const int z0 = 1 + tiz * J.tz, z1 = std::min(z0 + J.tz, J.nzbc - 1);
+    const int x0 = 1 + tix * out_cols, x1 = std::min(x0 + out_cols, J.nxbc - 1);
+    const double rows = z1 - z0 + 2 * NT + 1;
+    if (J.nsf <= 0) return rows;
+    const int ir = std::max(0, std::min(z1, J.pt + J.nz + 1) - std::max(z0, J.pt + 1));
+    const int ic = std::max(0, std::min(x1, J.nbc + J.nx) - std::max(x0, J.nbc));
+    const double per_launch = std::min(1.0, (double)NT / std::max(J.in_wf, 1));  // imaging levels per launch
+    return rows + (adj ? 0.65 : 0.1) * per_launch * ir * ((double)ic / out_cols);
+}
+
+// Workgroups the device holds at once for a time-blocked kernel: CUs x
+// resident workgroups per CU (0 if unknown).
+template <typename T>
+static long long tb_resident(bool adj, int NT, bool b1, int& cus)
+{
+    int dev = 0;
+    cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        cus = 0;
+        return 0;
+    }
+    return (long long)cus * tb_blocks_per_cu<T>(adj, NT, b1);
+}
+
+template <typename T>
+int build_march_plan(std::vector<FdJob<T>>& jobs, int tz, MarchPlan& plan, int out_cols, int NT, bool adj)
+{
+    std::vector<BlockMap> list;
     for (size_t j = 0; j < jobs.size(); ++j) {
         FdJob<T>& J = jobs[j];
         J.tz = tz;
         J.tiles_x = div_up(J.nxbc - 2, out_cols);
         J.tiles_z = div_up(J.nzbc - 2, tz);
         const int tiles = J.tiles_x * J.tiles_z;
-        for (int t = 0; t < tiles; t += kMarchWaves) plan.bmap.push_back(BlockMap{(int)j, t});
+        for (int t = 0; t < tiles; t += kMarchWaves) list.push_back(BlockMap{(int)j, t});
     }
-    plan.nblocks = (int)plan.bmap.size();
-    if (plan.nblocks == 0) { set_error("empty FD batch"); return kErrArg; }
+    const int n = (int)list.size();
+    plan.nblocks = n;
+    if (n == 0) { set_error("empty FD batch"); return kErrArg; }
+    if (NT <= 1) {   // single-step kernels: job-major
+        plan.bmap = std::move(list);
+        return kOk;
+    }
+    plan.bmap.resize(n);
+    int cus = 0;
+    const long long resident = tb_resident<T>(adj, NT, jobs_b1(jobs), cus);
+    if (!adj || cus < 8 || cus % 8 || n > resident) {
+        // the XCD grouping alone: for the forward, whose tiles differ little
+        // in work (balancing its CUs as below made it 1% slower), and when
+        // there are more workgroups than the device holds (the dispatcher
+        // refills CUs as workgroups end)
+        for (int i = 0; i < n; ++i) plan.bmap[i] = list[xcd_block(i, n)];
+        return kOk;
+    }
+    // Every workgroup resident at once: a CU then finishes with its
+    // heaviest workgroups and nothing rebalances them.  The dispatcher gives
+    // workgroup i to XCD i mod 8 and fills the CUs in rounds of one workgroup
+    // each, so workgroups i, i + cus, i + 2 cus, ... share a CU.  Workgroups
+    // sorted by estimated work (heaviest first, ties in list order) fill the
+    // rounds in snake order, so the CUs' sums even out; within a round,
+    // consecutive ones go to one XCD (one L2), as neighbouring tiles of a shot.
+    // Job-major with the XCD grouping only, the production adjoint's CUs
+    // finished between 61 and 103 us, in step with their share of imaging
+    // tiles.
+    std::vector<double> work(n, 0.0);
+    for (int k = 0; k < n; ++k)
+        for (int w = 0; w < kMarchWaves; ++w)
+            work[k] += tb_tile_work(jobs[list[k].job], list[k].tile0 + w, out_cols, NT, adj);
+    std::vector<int> order(n);
+    for (int k = 0; k < n; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+    const int per_xcd = cus / 8;
+    for (int i = 0; i < n; ++i) {
+        const int r = i / cus, c = i % cus, m = std::min(cus, n - r * cus);
+        // position in the round: XCD-major when the round is whole
+        const int p = m == cus ? (c % 8) * per_xcd + c / 8 : c;
+        plan.bmap[i] = list[order[r * cus + ((r & 1) ? m - 1 - p : p)]];
+    }
     return kOk;
 }
-template int build_march_plan<float>(std::vector<FdJob<float>>&, int, MarchPlan&, int);
-template int build_march_plan<double>(std::vector<FdJob<double>>&, int, MarchPlan&, int);
+template int build_march_plan<float>(std::vector<FdJob<float>>&, int, MarchPlan&, int, int, bool);
+template int build_march_plan<double>(std::vector<FdJob<double>>&, int, MarchPlan&, int, int, bool);
 
 template <typename T>
 int auto_band_rows(const std::vector<FdJob<T>>& jobs, bool adj, int NT)
@@ -105,7 +184,7 @@ static int tb_segments(const std::vector<FdJob<T>>& jobs, bool adj, int NT, int 
         for (size_t j = 0; j < jobs.size(); ++j)
             if (last[j] > L0) sg.jobs.push_back(jobs[j]);
         sg.rows = (L0 == 0 && tb_rows > 0) ? tb_rows : auto_band_rows(sg.jobs, adj, NT);
-        if ((rc = build_march_plan(sg.jobs, sg.rows, sg.plan, kMarchLanes - 2 * NT))) return rc;
+        if ((rc = build_march_plan(sg.jobs, sg.rows, sg.plan, kMarchLanes - 2 * NT, NT, adj))) return rc;
         segs.push_back(std::move(sg));
         L0 = c;
     }

@@ -645,7 +645,7 @@ int Server::rebuild(int ph)
     if ((int)jobs.size() > kMaxJobs) { set_error("more than %d shots in one FD launch", kMaxJobs); return kErrArg; }
     const int rows = auto_band_rows(jobs, ph == 1, NT);
     MarchPlan plan;
-    int rc = build_march_plan(jobs, rows, plan, kMarchLanes - 2 * NT);
+    int rc = build_march_plan(jobs, rows, plan, kMarchLanes - 2 * NT, NT, ph == 1);
     if (rc) return rc;
     if (plan.nblocks > kMaxBlocks) { set_error("FD launch of %d workgroups exceeds %d", plan.nblocks, kMaxBlocks); return kErrArg; }
     const int k = stage_next[ph];
