@@ -12,15 +12,24 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <new>
+#include <thread>
 
 namespace swd {
 
 Engine::~Engine()
 {
     if (st) (void)hipStreamSynchronize(st);
+    for (auto& b : batch_fd)
+        if (b) {
+            (void)hipStreamSynchronize(b);
+            (void)hipStreamDestroy(b);
+        }
     if (own_stream && st) (void)hipStreamDestroy(st);
 }
 
@@ -369,6 +378,228 @@ int run_fd_batch(const std::vector<Engine*>& engines, bool adj, bool snapshots, 
             else { e->times.fwd_spl = ms.nt_block; e->times.fwd_rows = ms.rows; }
         }
     if (stats) *stats = ms;
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Uneven batch with overlapped phases
+// ---------------------------------------------------------------------------
+namespace {
+
+// One launch table of a batch phase: the jobs of the models it advances and
+// their block map, kept alive until every launch that reads it has run.
+struct BatchTable {
+    DevBuf djobs, dbmap;
+    int nblocks = 0;
+    bool b1 = false;
+};
+
+// Launches of one phase go in chunks with an event after each; a chunk is
+// issued only once the chunk two before it has run, so about two chunks are
+// queued ahead of the device, and a model joins or leaves within two chunks
+// of the moment it can.
+struct Pacer {
+    static constexpr int kChunk = 8;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool used[2] = {false, false};
+    int k = 0;
+    Pacer() = default;
+    Pacer(const Pacer&) = delete;
+    Pacer& operator=(const Pacer&) = delete;
+    ~Pacer()
+    {
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    int create()
+    {
+        for (auto& e : ev) SWD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return kOk;
+    }
+    // 1: a chunk may be issued, 0: not yet, < 0: error
+    int ready()
+    {
+        if (!used[k]) return 1;
+        const hipError_t e = hipEventQuery(ev[k]);
+        if (e == hipSuccess) return 1;
+        if (e == hipErrorNotReady) return 0;
+        set_error("event query failed: %s", hipGetErrorString(e));
+        return kErrHip;
+    }
+    int issued(hipStream_t st)
+    {
+        SWD_HIP(hipEventRecord(ev[k], st));
+        used[k] = true;
+        k ^= 1;
+        return kOk;
+    }
+};
+
+// 1: the event has completed, 0: not yet, < 0: error
+int event_done(hipEvent_t e)
+{
+    const hipError_t q = hipEventQuery(e);
+    if (q == hipSuccess) return 1;
+    if (q == hipErrorNotReady) return 0;
+    set_error("event query failed: %s", hipGetErrorString(q));
+    return kErrHip;
+}
+
+}  // namespace
+
+// The FD phases of the models overlap: the forward launches advance every
+// model until its last step, and as soon as a model's forward has run, its
+// Radon / pick / adjoint-source stage runs on its own stream and its adjoint
+// jobs join the adjoint launches at their next chunk, while longer models are
+// still in their forward.  Its assembly follows its last adjoint launch.
+// Done one phase after the other, the batch leaves the GPU part idle while
+// only the longest models remain in a phase: the 8-model uneven batch took
+// 380 ms that way, 345 ms through the serving loop, which overlaps requests.
+// A dependency between streams is enqueued only once its event has completed
+// (the host polls), so no stream holds a barrier that could stall the others
+// sharing its hardware queue.  Launch tables change only at chunk
+// boundaries; a joining model's job origin (start, launch0) is the adjoint
+// launch it joins at, so its steps and ping-pong parity are those it would
+// have alone, and results are bitwise those of each model's own gradient.
+int batch_gradient_overlapped(const std::vector<Engine*>& v, double** dk_vs, double* kernel_us)
+{
+    const int n = (int)v.size();
+    Engine* e0 = v[0];
+    const int NT[2] = {e0->nt_block[0], e0->nt_block[1]};
+    if (NT[0] < 2 || NT[1] < 2) { set_error("overlapped batch needs time-blocked launches"); return kErrArg; }
+    for (auto& b : e0->batch_fd)
+        if (!b) SWD_HIP(hipStreamCreateWithFlags(&b, hipStreamNonBlocking));
+    hipStream_t fs[2] = {e0->batch_fd[0], e0->batch_fd[1]};
+    int rc;
+    std::vector<std::vector<FdJob<float>>> jobs[2];
+    std::vector<int> nl[2];   // launches of each model per phase
+    for (int ph = 0; ph < 2; ++ph) jobs[ph].resize(n);
+    for (int m = 0; m < n; ++m) {
+        Engine* x = v[m];
+        x->make_jobs(jobs[0][m], false, true);
+        x->make_jobs(jobs[1][m], true, true);
+        nl[0].push_back(div_up(x->p.nt, NT[0]));
+        nl[1].push_back(div_up(std::max(x->p.nt - 1, 0), NT[1]));
+        SWD_HIP(hipMemsetAsync(x->state.p, 0, x->state.bytes, fs[0]));
+        SWD_HIP(hipMemsetAsync(x->traces.p, 0, x->traces.bytes, fs[0]));
+    }
+    EventSet ev(3 * (size_t)n);   // per model: forward run, stage run, adjoint run
+    EventSet span(4);             // per phase: before its first launch, after its latest
+    Pacer pace[2];
+    if ((rc = ev.create()) || (rc = span.create()) || (rc = pace[0].create()) || (rc = pace[1].create())) return rc;
+    enum { kFwd, kAfterFwd, kStage, kAdj, kAfterAdj, kDone };
+    std::vector<int> at(n, kFwd), join(n, 0);
+    std::vector<std::unique_ptr<BatchTable>> keep;
+    BatchTable* tab[2] = {nullptr, nullptr};
+    bool dirty[2] = {true, true};
+    int g[2] = {0, 0};   // launches issued per phase
+    int left = n;
+    // errors only stop the loop: everything queued drains before the tables go
+    rc = kOk;
+    auto hip = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess) {
+            set_error("%s failed: %s", what, hipGetErrorString(e));
+            rc = kErrHip;
+        }
+        return rc == kOk;
+    };
+    while (rc == kOk && left > 0) {
+        bool moved = false;
+        for (int m = 0; m < n && rc == kOk; ++m) {
+            Engine* x = v[m];
+            int d = 0;
+            if (at[m] == kAfterFwd && (d = event_done(ev.ev[3 * m])) > 0) {
+                // the model's forward has run: its stage, and the adjoint's initial state
+                if (!hip(hipStreamWaitEvent(x->st, ev.ev[3 * m], 0), "hipStreamWaitEvent") ||
+                    !hip(hipMemsetAsync(x->state.p, 0, x->state.bytes, x->st), "hipMemsetAsync") ||
+                    !hip(hipMemsetAsync(x->imgs.p, 0, x->imgs.bytes, x->st), "hipMemsetAsync") ||
+                    !hip(hipMemsetAsync(x->imgp.p, 0, x->imgp.bytes, x->st), "hipMemsetAsync"))
+                    break;
+                if ((rc = x->dispersion()) || (rc = x->adjoint_source()) || (rc = ev.record(3 * m + 1, x->st))) break;
+                at[m] = kStage;
+                moved = true;
+            } else if (at[m] == kStage && (d = event_done(ev.ev[3 * m + 1])) > 0) {
+                join[m] = g[1];
+                for (auto& J : jobs[1][m]) {
+                    J.start = g[1] * NT[1];
+                    J.launch0 = g[1];
+                }
+                at[m] = kAdj;
+                dirty[1] = true;
+                moved = true;
+                if (nl[1][m] == 0) at[m] = kAfterAdj, rc = ev.record(3 * m + 2, x->st);
+            } else if (at[m] == kAfterAdj && (d = event_done(ev.ev[3 * m + 2])) > 0) {
+                if (!hip(hipStreamWaitEvent(x->st, ev.ev[3 * m + 2], 0), "hipStreamWaitEvent") ||
+                    (rc = x->assemble()))
+                    break;
+                if (dk_vs && dk_vs[m] &&
+                    !hip(hipMemcpyAsync(dk_vs[m], x->outs.p, x->splane * sizeof(double), hipMemcpyDeviceToHost, x->st),
+                         "hipMemcpyAsync"))
+                    break;
+                at[m] = kDone;
+                --left;
+                moved = true;
+            }
+            if (d < 0) rc = d;
+        }
+        for (int ph = 0; ph < 2 && rc == kOk; ++ph) {
+            const int in = ph ? kAdj : kFwd;
+            std::vector<int> mem;
+            for (int m = 0; m < n; ++m)
+                if (at[m] == in) mem.push_back(m);
+            if (mem.empty()) continue;
+            const int r = pace[ph].ready();
+            if (r <= 0) {
+                rc = r;
+                continue;
+            }
+            if (dirty[ph]) {
+                std::vector<FdJob<float>> js;
+                for (int m : mem) js.insert(js.end(), jobs[ph][m].begin(), jobs[ph][m].end());
+                const int rows = (ph == 0 && g[0] == 0 && e0->tb_rows[0] > 0) ? e0->tb_rows[0]
+                                                                              : auto_band_rows(js, ph == 1, NT[ph]);
+                auto t = std::make_unique<BatchTable>();
+                MarchPlan plan;
+                if ((rc = build_march_plan(js, rows, plan, kMarchLanes - 2 * NT[ph], NT[ph], ph == 1)) ||
+                    (rc = t->djobs.upload(js.data(), js.size() * sizeof(FdJob<float>), fs[ph])) ||
+                    (rc = t->dbmap.upload(plan.bmap.data(), plan.bmap.size() * sizeof(BlockMap), fs[ph])))
+                    break;
+                t->nblocks = plan.nblocks;
+                t->b1 = jobs_b1(js);
+                tab[ph] = t.get();
+                keep.push_back(std::move(t));
+                dirty[ph] = false;
+            }
+            int next = INT_MAX;   // the next launch after which a model leaves
+            for (int m : mem) next = std::min(next, (ph ? join[m] : 0) + nl[ph][m]);
+            const int end = std::min(next, g[ph] + Pacer::kChunk);
+            const BatchTable& T = *tab[ph];
+            if (g[ph] == 0 && (rc = span.record(2 * ph, fs[ph]))) break;
+            for (; g[ph] < end && rc == kOk; ++g[ph])
+                rc = ph ? launch_adj_tb<float>(fs[1], T.djobs.as<FdJob<float>>(), T.dbmap.as<BlockMap>(), T.nblocks,
+                                               g[1] * NT[1], g[1], NT[1], T.b1)
+                        : launch_fwd_tb<float>(fs[0], T.djobs.as<FdJob<float>>(), T.dbmap.as<BlockMap>(), T.nblocks,
+                                               g[0] * NT[0], g[0], NT[0], T.b1);
+            if (rc || (rc = span.record(2 * ph + 1, fs[ph])) || (rc = pace[ph].issued(fs[ph]))) break;
+            for (int m : mem)
+                if ((ph ? join[m] : 0) + nl[ph][m] == g[ph]) {
+                    if ((rc = ev.record(3 * m + 2 * ph, fs[ph]))) break;
+                    at[m] = ph ? kAfterAdj : kAfterFwd;
+                    dirty[ph] = true;
+                }
+            moved = true;
+        }
+        if (!moved && rc == kOk) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+    for (auto s : fs) (void)hipStreamSynchronize(s);
+    for (Engine* x : v) (void)hipStreamSynchronize(x->st);
+    if (kernel_us)
+        for (int ph = 0; ph < 2; ++ph) {
+            float ms = 0;
+            if (rc != kOk || g[ph] == 0 || hipEventElapsedTime(&ms, span.ev[2 * ph], span.ev[2 * ph + 1]) != hipSuccess)
+                ms = 0;
+            kernel_us[ph] = g[ph] ? 1000.0 * ms / g[ph] : 0.0;
+        }
     return rc;
 }
 
@@ -760,6 +991,9 @@ int swd_batch_gradient(swd_engine** es, int n, double** dk_vs, double* kernel_us
     hipStream_t st = v[0]->st;
     // every engine's stream must be idle before the shared launches
     for (Engine* x : v) SWD_HIP(hipStreamSynchronize(x->st));
+    bool overlap = v[0]->nt_block[0] >= 2 && v[0]->nt_block[1] >= 2;
+    for (Engine* x : v) overlap = overlap && x->fd_path == 0;
+    if (overlap) return batch_gradient_overlapped(v, dk_vs, kernel_us);
     MarchStats us[2];
     int rc = run_fd_batch(v, false, true, st, v[0]->tz, &us[0]);
     if (rc) return rc;

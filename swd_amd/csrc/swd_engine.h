@@ -47,6 +47,7 @@ public:
     bool b1_same = false;  // the b1 plane equals b (jobs then read b for b1)
     bool use_graph = false;    // FD phases replay captured HIP graphs
     MarchGraph graphs[2][2][2];   // [adjoint][snapshots][multi-engine batch]
+    hipStream_t batch_fd[2] = {nullptr, nullptr};  // forward / adjoint launch streams of batches this engine leads
 
     DevBuf coefs;     // 6 planes float: temp, b, b1, ca, cl, cm1
     DevBuf vs_d;      // nz x nx double (MATLAB layout) for the assembly
@@ -101,6 +102,13 @@ public:
 private:
     int run_fd(bool adjoint, bool snapshots);
 };
+
+// Gradient of several engines (models of different grid and nt) with their
+// FD phases overlapped; dk_vs[i] (nz x nx, may be null) receives engine i's
+// gradient, kernel_us (may be null) each phase's event-timed span divided by
+// its launch count.  Needs time-blocked launch shapes (nt_block >= 2) in
+// both phases.
+int batch_gradient_overlapped(const std::vector<Engine*>& engines, double** dk_vs, double* kernel_us);
 
 // Run the FD phase of several engines in shared launches (one launch covers
 // every shot of every engine; engines may differ in grid size and nt) with

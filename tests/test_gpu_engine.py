@@ -286,3 +286,37 @@ def test_uneven_batch_autotuned_is_the_same_alone_and_in_mix():
         assert err < 1e-4
     for e in engs:
         e.close()
+
+
+def test_overlapped_batch_equals_phase_by_phase_batch_and_alone():
+    """The overlapped batch schedule (forward and adjoint launch sequences on
+    two streams, each model's adjoint joining as soon as its forward and its
+    dispersion stage have run) against the phase-by-phase batch (single-step
+    launches, all forwards then all adjoints) and against each model alone:
+    bitwise the same gradients, on three models of different grid and nt
+    whose forwards end at different launches."""
+    from swd_amd.engine import batch_gradient
+    models = P.uneven_models(shots=2, sizes=(200, 300, 450), nt_range=(1500, 2500))
+    engs = []
+    for prob, mod in models:
+        e = Engine(prob)
+        e.set_wavelet(mod["s"])
+        e.set_model(mod["vp_true"], mod["vs_true"])
+        e.observe()
+        e.set_model(mod["vp"], mod["vs"])
+        engs.append(e)
+    assert len({p["nt"] for p, _ in models}) == 3
+    for e in engs:
+        e.set_time_block(5, 0)
+    ovl, us = batch_gradient(engs)
+    assert us[0] > 0 and us[1] > 0
+    alone = [e.gradient()["dk_vs"] for e in engs]
+    for e in engs:
+        e.set_time_block(1, 0)
+    seq, _ = batch_gradient(engs)
+    for a, b, c in zip(ovl, seq, alone):
+        assert np.isfinite(a).mean() > 0.5
+        np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(a, c)
+    for e in engs:
+        e.close()

@@ -70,9 +70,11 @@ def test_graph_replays_match_eager():
 
 def test_batch_graph_replays_match_eager():
     """swd_batch_gradient over uneven models (different grids and nt) with
-    graph replay: every model's gradient bitwise equal to the eager batch, on
-    repeated replays with eager batches between them and after one model's
-    change; the single-engine graphs of the first engine stay valid."""
+    the first engine's graph setting on and off: every model's gradient
+    bitwise equal across repeated batches, with graph-replayed single-engine
+    gradients of the first engine between them and after one model's change
+    (time-blocked batches run the overlapped schedule, eagerly; the
+    single-engine graphs stay valid around it)."""
     from swd_amd.engine import batch_gradient
     models = P.uneven_models(shots=2, sizes=(200, 300, 450), nt_range=(1500, 2500))
     engs = []
