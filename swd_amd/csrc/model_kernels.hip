@@ -126,6 +126,24 @@ __global__ __launch_bounds__(1024) void model_stats_kernel(const double* __restr
     }
 }
 
+// temp, ca, cl, cm1 of a coefficient block (planes temp, b, b1, ca, cl, cm1)
+// interleaved per cell: the same values, one 16-byte record per cell
+__global__ void pack_coef4_kernel(const float* __restrict__ c, size_t plane, float4* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= plane) return;
+    out[i] = make_float4(c[i], c[3 * plane + i], c[4 * plane + i], c[5 * plane + i]);
+}
+
+int launch_pack_coef4(hipStream_t st, const float* coefs, size_t plane, float* out)
+{
+    if ((reinterpret_cast<uintptr_t>(out) & 15) != 0) { set_error("interleaved coefficients need 16-byte alignment"); return kErrArg; }
+    const int blk = 256;
+    pack_coef4_kernel<<<(unsigned)((plane + blk - 1) / blk), blk, 0, st>>>(coefs, plane, (float4*)out);
+    SWD_HIP(hipGetLastError());
+    return kOk;
+}
+
 int launch_prep_coefs(hipStream_t st, const double* vp, const double* vs, const double* geo, double kappa, int nz,
                       int nx, int nbc, int top, int nzbc, int nxbc, int pitch, size_t plane, double dt, double dtx,
                       float* coefs)

@@ -114,7 +114,7 @@ int Engine::create(const swd_problem* prob, hipStream_t stream)
     np = disp.g.np;
     // device buffers
     const size_t S = p.nshots, ntr = (size_t)p.nt * p.ng;
-    if ((rc = coefs.alloc(6 * plane * sizeof(float))) || (rc = vs_d.alloc(splane * sizeof(double))) ||
+    if ((rc = coefs.alloc(10 * plane * sizeof(float))) || (rc = vs_d.alloc(splane * sizeof(double))) ||
         (rc = wavelet.alloc(p.nt * sizeof(float))) || (rc = state.alloc(S * 10 * plane * sizeof(float))) ||
         (rc = traces.alloc(S * ntr * sizeof(float))) ||
         (rc = snaps.alloc(S * (size_t)nrec * 3 * splane * sizeof(float))) ||
@@ -173,6 +173,7 @@ int Engine::set_model(const double* vp, const double* vs)
     }
     b1_same = std::equal(h.begin() + plane, h.begin() + 2 * plane, h.begin() + 2 * plane);
     SWD_HIP(hipMemcpyAsync(coefs.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    if ((rc = launch_pack_coef4(st, coefs.as<float>(), plane, coefs.as<float>() + 6 * plane))) return rc;
     SWD_HIP(hipMemcpyAsync(vs_d.p, vs, splane * sizeof(double), hipMemcpyHostToDevice, st));
     SWD_HIP(hipMemcpyAsync(vp_d.p, vp, splane * sizeof(double), hipMemcpyHostToDevice, st));
     SWD_HIP(hipStreamSynchronize(st));
@@ -197,7 +198,8 @@ int Engine::set_vs_device(const double* vs_dev)
     vpmax = vmax;
     const double kappa = damp_kappa(vmin, p.nbc, p.dx);
     if ((rc = launch_prep_coefs(st, vp_d.as<double>(), vs_dev, geo.as<double>(), kappa, p.nz, p.nx, p.nbc, pt_m, nzbc,
-                                nxbc, pitch, plane, p.dt, p.dt / p.dx, coefs.as<float>())))
+                                nxbc, pitch, plane, p.dt, p.dt / p.dx, coefs.as<float>())) ||
+        (rc = launch_pack_coef4(st, coefs.as<float>(), plane, coefs.as<float>() + 6 * plane)))
         return rc;
     b1_same = true;  // prep_coefs_kernel writes b1 = b
     if (vs_dev != vs_d.as<double>())
@@ -332,7 +334,7 @@ void Engine::make_jobs(std::vector<FdJob<float>>& jobs, bool adj, bool snapshots
             J.xz[b] = S + (5 * b + 4) * plane;
         }
         J.temp = C; J.b = C + plane; J.b1 = b1_same ? J.b : C + 2 * plane; J.ca = C + 3 * plane;
-        J.cl = C + 4 * plane; J.cm1 = C + 5 * plane;
+        J.cl = C + 4 * plane; J.cm1 = C + 5 * plane; J.c4 = C + 6 * plane;
         const size_t ntr = (size_t)p.nt * p.ng;
         J.src = adj ? adjsrc.as<float>() + s * ntr : wavelet.as<float>();
         J.trace_w = adj ? nullptr : traces.as<float>() + s * ntr;

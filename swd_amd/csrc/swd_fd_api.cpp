@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "swd_abi.h"
@@ -137,6 +138,24 @@ static bool fwd_fusable(const FwdDesc& d)
 // ---------------------------------------------------------------------------
 // one-shot forward (MEX-compatible)
 // ---------------------------------------------------------------------------
+// The device coefficient block (planes temp, b, b1, ca, cl, cm1); for float
+// it also holds temp, ca, cl, cm1 interleaved per cell (FdJob::c4), which the
+// time-blocked kernels read.
+template <typename T>
+static int upload_coefs(DevBuf& coefs, const std::vector<T>& hc, size_t plane, hipStream_t st, const T*& c4)
+{
+    int rc;
+    c4 = nullptr;
+    if constexpr (std::is_same<T, float>::value) {
+        if ((rc = coefs.alloc(10 * plane * sizeof(float)))) return rc;
+        SWD_HIP(hipMemcpyAsync(coefs.p, hc.data(), hc.size() * sizeof(float), hipMemcpyHostToDevice, st));
+        if ((rc = launch_pack_coef4(st, (const float*)coefs.p, plane, (float*)coefs.p + 6 * plane))) return rc;
+        c4 = (const T*)coefs.p + 6 * plane;
+        return kOk;
+    }
+    return coefs.upload(hc.data(), hc.size() * sizeof(T), st);
+}
+
 template <typename T>
 static int staggerfd_impl(const T* iv, const T* const coef_in[7], const T* s, T* seismo_u, T* seismo_w, T* snap_out[4])
 {
@@ -164,7 +183,8 @@ static int staggerfd_impl(const T* iv, const T* const coef_in[7], const T* s, T*
         pack_grid(coef_in[map[k]], d.nzbc, d.nxbc, P, hc.data() + k * plane);
         if (fused) zero_ring(hc.data() + k * plane, d.nzbc, d.nxbc, P);
     }
-    if ((rc = coefs.upload(hc.data(), hc.size() * sizeof(T), st.s))) return rc;
+    const T* c4 = nullptr;
+    if ((rc = upload_coefs(coefs, hc, plane, st.s, c4))) return rc;
     if ((rc = state.alloc(10 * plane * sizeof(T))) || (rc = traces.alloc(2 * ntr * sizeof(T))) ||
         (rc = src.upload(s, (size_t)d.nt * sizeof(T), st.s)))
         return rc;
@@ -185,6 +205,7 @@ static int staggerfd_impl(const T* iv, const T* const coef_in[7], const T* s, T*
         J.xz[b] = S + (5 * b + 4) * plane;
     }
     J.temp = C; J.b = C + plane; J.b1 = C + 2 * plane; J.ca = C + 3 * plane; J.cl = C + 4 * plane; J.cm1 = C + 5 * plane;
+    J.c4 = c4;
     // b1 equal to b (as the wrappers make it): the fused kernels then read b for both
     if (fused && std::equal(hc.begin() + plane, hc.begin() + 2 * plane, hc.begin() + 2 * plane)) J.b1 = J.b;
     J.src = (const T*)src.p;
@@ -291,7 +312,8 @@ static int e2drtm_impl(const T* iv, const T* const coef_in[7], const T* adj, con
         pack_grid(coef_in[map[k]], d.nzbc, d.nxbc, P, hc.data() + k * plane);
         if (fused) zero_ring(hc.data() + k * plane, d.nzbc, d.nxbc, P);
     }
-    if ((rc = coefs.upload(hc.data(), hc.size() * sizeof(T), st.s))) return rc;
+    const T* c4 = nullptr;
+    if ((rc = upload_coefs(coefs, hc, plane, st.s, c4))) return rc;
     {
         std::vector<T> h(ntr);
         for (int it = 0; it < d.nt; ++it)
@@ -339,6 +361,7 @@ static int e2drtm_impl(const T* iv, const T* const coef_in[7], const T* adj, con
         J.xz[b] = S + (5 * b + 4) * plane;
     }
     J.temp = C; J.b = C + plane; J.b1 = C + 2 * plane; J.ca = C + 3 * plane; J.cl = C + 4 * plane; J.cm1 = C + 5 * plane;
+    J.c4 = c4;
     // b1 equal to b (as the wrappers make it): the fused kernels then read b for both
     if (fused && std::equal(hc.begin() + plane, hc.begin() + 2 * plane, hc.begin() + 2 * plane)) J.b1 = J.b;
     J.src = (const T*)dadj.p;

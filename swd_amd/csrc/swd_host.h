@@ -37,6 +37,9 @@ int launch_smooth2a(hipStream_t st, const double* in, int nz, int nx, int Nr, in
 int launch_model_step(hipStream_t st, const double* vs, const double* dk, double step, double vsmin, double vsmax,
                       double* out, size_t n);
 int launch_model_scale(hipStream_t st, const double* vs, double ratio, double* vp, size_t n);
+// out (4 x plane floats, 16-byte aligned) = temp, ca, cl, cm1 of the coefficient
+// block interleaved per cell (FdJob::c4)
+int launch_pack_coef4(hipStream_t st, const float* coefs, size_t plane, float* out);
 int launch_model_stats(hipStream_t st, const double* a, const double* b, size_t n, double* out);
 
 // First-fit sub-allocator over one allocation made once (device memory via

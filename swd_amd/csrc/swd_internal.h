@@ -54,6 +54,10 @@ struct FdJob {
     const T* ca;
     const T* cl;
     const T* cm1;
+    // float jobs: temp, ca, cl, cm1 interleaved per cell ([row][pitch][4], 16 B
+    // per cell), which the time-blocked kernels read with one 16-byte load per
+    // lane instead of four; null for double jobs
+    const T* c4;
     // forward: wavelet s[nt]; adjoint: adjoint source [nt][ng] (g fastest)
     const T* src;
     // forward receivers [nt][ng] (g fastest); trace_u may be null

@@ -200,6 +200,8 @@ int run_march(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int steps, 
         if (adj && std::is_same<T, float>::value && !J.imgp) { set_error("float adjoint jobs need partial images"); return kErrArg; }
         if (J.stype != 0 && J.stype != 5) tb_ok = false;
         if (adj && J.nsf != 3) tb_ok = false;   // the time-blocked adjoint reads the 3-field snapshots
+        // float jobs need the interleaved coefficients, addressed with 32-bit offsets
+        if (std::is_same<T, float>::value && (!J.c4 || (size_t)J.nzbc * J.pitch * 16 > 0xffffffffull)) tb_ok = false;
         if (J.start != 0 || J.launch0 != 0) { set_error("run_march jobs must start together"); return kErrArg; }
     }
     const int NT = tb_ok ? nt_block : 1;
@@ -297,6 +299,8 @@ int run_march_graph(hipStream_t st, std::vector<FdJob<T>>& jobs, bool adj, int s
         if (adj && std::is_same<T, float>::value && !J.imgp) { set_error("float adjoint jobs need partial images"); return kErrArg; }
         if (J.stype != 0 && J.stype != 5) tb_ok = false;
         if (adj && J.nsf != 3) tb_ok = false;   // the time-blocked adjoint reads the 3-field snapshots
+        // float jobs need the interleaved coefficients, addressed with 32-bit offsets
+        if (std::is_same<T, float>::value && (!J.c4 || (size_t)J.nzbc * J.pitch * 16 > 0xffffffffull)) tb_ok = false;
         if (J.start != 0 || J.launch0 != 0) { set_error("run_march jobs must start together"); return kErrArg; }
     }
     const int NT = tb_ok ? nt_block : 1;
