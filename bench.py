@@ -161,8 +161,8 @@ def measured_profile(kname, shape, args):
     with open(path) as fh:
         t = json.load(fh)
     k = kname.replace("_march", "_tb") if shape[0] > 1 else kname
-    rec = t.get("kernels", {}).get(k)
-    if rec is None or list(t.get("shapes", {}).get(k, [])) != list(shape):
+    rec = t.get("by_shape", {}).get(f"{k}:{shape[0]}x{shape[1]}")
+    if rec is None:
         return {}
     return {"bytes_per_launch": int(rec["bytes_per_launch"]), "valu_insts": rec.get("valu_insts")}
 

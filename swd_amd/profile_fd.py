@@ -14,7 +14,9 @@ reads the counter CSVs, scales the FD kernels' FETCH_SIZE / WRITE_SIZE by the
 copy kernel's (bytes known), and writes the per-launch HBM traffic of each
 time-blocked kernel, and its VALU instruction count per launch when the
 ``sq`` pass is there, to ``fd_traffic.json`` beside this file (read by the
-benchmark) and a markdown table to stdout.
+benchmark; records of earlier runs at other launch shapes are kept, keyed by
+kernel and shape, e.g. ``fd_fwd_tb:6x26``) and a markdown table to stdout.
+``--shapes`` names the shapes of the run (default: both phases at 5 x 26).
 """
 from __future__ import annotations
 
@@ -126,6 +128,17 @@ def summarize(args):
                 lines.append(f"| {s} | {n} | VALU instructions per dispatch {v / 1e6:.2f} M | | |")
     if args.shapes:
         res["shapes"] = json.loads(args.shapes)
+    # records by kernel and launch shape, merged with those of earlier runs at
+    # other shapes (the benchmark looks up the shape its autotuner picked)
+    by_shape = {}
+    if os.path.exists(TRAFFIC_JSON):
+        with open(TRAFFIC_JSON) as fh:
+            by_shape = json.load(fh).get("by_shape", {})
+    for k, rec in res["kernels"].items():
+        shp = res.get("shapes", {}).get(k)
+        if shp:
+            by_shape[f"{k}:{shp[0]}x{shp[1]}"] = dict(rec, copy_fetch_kib=cf[0][0], copy_write_kib=cw[0][0])
+    res["by_shape"] = by_shape
     with open(TRAFFIC_JSON, "w") as fh:
         json.dump(res, fh, indent=1)
     print(f"calibration: copy of {COPY_BYTES} B reads {cf[0][0]:.0f} KiB (scale {f_fetch:.4f}), "
