@@ -16,6 +16,10 @@ invariants elsewhere.
   models): every served result is bitwise the result of the request alone.
 - Dispersion path alone (60 chirp gathers of 6000 x 600): picks within one
   velocity bin of the FP64 oracle's, adjoint sources at beta 10 within 1e-4.
+- The reference's CPU-runnable single shot (60 x 200 cells, 82 x 240 padded,
+  nt 3000, 199 receivers): observed picks, gather, picks, misfit, adjoint
+  source and dk_vs against the oracle (the chain after the picks from the
+  oracle's picks wherever an FP32 pick lands one bin away).
 
 FP32 tolerance: relative L2 <= 1e-4 against FP64 for gathers, adjoint
 sources, images and gradients; picks within one velocity bin (1 m/s)."""
@@ -278,3 +282,56 @@ def test_dispersion_60_shots_vs_oracle():
     print(f"[dispersion] 60 shots: {n_within} of {n_cols} picks within one bin; worst adjoint-source rel-L2 "
           f"{worst_adj:.2e}; adjoint batch {ms:.2f} ms")
     assert worst_adj < 1e-4
+
+
+def test_single_shot_workload_full_size_vs_oracle():
+    """The single-shot workload at its stated size (nt 3000): the device's
+    observed picks (SWD.m:150-162) within one bin of the oracle's; the
+    gather, the adjoint source and dk_vs within 1e-4 of FP64; picks within
+    one bin, and where any pick differs the stages after the picks are judged
+    from the oracle's picks."""
+    prob, mod = P.plain_single_shot(nt=3000)
+    q = prob
+    gx = np.arange(q["gx0"], q["gx0"] + q["ng"], dtype=float)
+    gz = np.full(q["ng"], float(q["gz"]))
+    radon = (q["df"], q["dt"], 701, q["vmin"], q["vmax"], q["fmin"], q["fmax"], q["nt"], q["ng"], q["ds"],
+             q["offset"], q["m"], q["M"])
+    _, obs, _ = O.staggerfd_eigend(int(q["is_index"][0]), q["nbc"], q["nt"], q["dt"] / q["dx"], q["dx"], q["dt"],
+                                   int(q["sx"][0]), int(q["sz"][0]), gx, gz, mod["s"], mod["vp_true"],
+                                   mod["vs_true"], 1, 22, "w", 0, q["in_wf"], snapshots=False)
+    cr0_r = O.pickFDC1(O.rt_side(obs, "r", 5, *radon)[0], q["ini"], 2, 4) + q["vmin"]
+    cr0_l = O.pickFDC1(O.rt_side(obs, "l", 5, *radon)[0], q["ini"], 2, 4) + q["vmin"]
+    rad = dict(df=q["df"], fmin=q["fmin"], fmax=q["fmax"], vmin=q["vmin"], vmax=q["vmax"], ds=q["ds"],
+               offset=q["offset"], m=q["m"], M=q["M"])
+    o = O.swd_shot(mod["vp"], mod["vs"], mod["s"], q["nbc"], q["nt"], q["dx"], q["dt"], int(q["sx"][0]),
+                   int(q["sz"][0]), gx, gz, q["in_wf"], 22, 5, 5, rad, cr0_r, cr0_l, q["beta"], q["ini"], q["w"])
+    dk_ref = O.assemble_dk_vs(mod["vs"], o["cl_img"], o["cm_img"], o["illum"])
+
+    e = Engine(prob)
+    e.set_wavelet(mod["s"])
+    e.set_model(mod["vp_true"], mod["vs_true"])
+    a, b = e.observe()
+    d_obs = max(np.max(np.abs(a[:, 0] - cr0_r)), np.max(np.abs(b[:, 0] - cr0_l)))
+    e.set_model(mod["vp"], mod["vs"])
+    e.set_observed_picks(cr0_r[:, None], cr0_l[:, None])
+    g = e.gradient()
+    err = {"gather": rel(e.read("gathers")[0], o["seismo_w"])}
+    dr = np.max(np.abs(g["cr_r"][:, 0] - o["cr_r"]))
+    dl = np.max(np.abs(g["cr_l"][:, 0] - o["cr_l"]))
+    if dr == 0 and dl == 0:
+        err["adjoint_source"] = rel(e.read("adjoint_sources")[0], o["adj"])
+        err["dk_vs"] = rel(g["dk_vs"], dk_ref)
+        assert g["res_r"][0] == pytest.approx(o["res_r"], rel=1e-9)
+    else:
+        e.run("forward")
+        e.run("dispersion")
+        e.set_picks(o["cr_r"][:, None], o["cr_l"][:, None])
+        e.run("adjoint_source")
+        e.run("adjoint")
+        e.run("assemble")
+        err["adjoint_source_given_oracle_picks"] = rel(e.read("adjoint_sources")[0], o["adj"])
+        err["dk_vs_given_oracle_picks"] = rel(e.read("gradient")[0], dk_ref)
+    e.close()
+    print(f"single shot nt 3000: observed picks max diff {d_obs:g} m/s, picks {dr:g} / {dl:g} m/s, rel-L2 {err}")
+    assert d_obs <= 1.0 and dr <= 1 and dl <= 1
+    assert max(err.values()) < 1e-4, err
