@@ -135,14 +135,19 @@ def cpu_baseline(prob, mod, max_threads=16):
 
 
 def copy_peak(nbytes=1 << 30, reps=10):
-    """HBM rate of a plain float4 device copy of 1 GiB (read + write bytes per
-    second), measured now: the practical ceiling beside the 8 TB/s figure."""
+    """HBM rates (read + write bytes per second) of two device copies of
+    1 GiB, measured now: the best streaming copy (swd_copy_peak, the practical
+    ceiling beside the 8 TB/s figure) and the plain float4 copy whose counter
+    readings calibrate the traffic figures (swd_calib_copy)."""
     import ctypes
     from swd_amd._lib import check, load
-    ms = ctypes.c_double(0)
-    check(load().swd_calib_copy(nbytes, 2, ctypes.byref(ms)))
-    check(load().swd_calib_copy(nbytes, reps, ctypes.byref(ms)))
-    return 2.0 * nbytes / (ms.value * 1e-3) / 1e9
+    out = []
+    for fn in (load().swd_copy_peak, load().swd_calib_copy):
+        ms = ctypes.c_double(0)
+        check(fn(nbytes, 2, ctypes.byref(ms)))
+        check(fn(nbytes, reps, ctypes.byref(ms)))
+        out.append(2.0 * nbytes / (ms.value * 1e-3) / 1e9)
+    return out
 
 
 N_SIMD = 1024           # MI355X: 256 CUs x 4 SIMDs
@@ -472,7 +477,7 @@ def main():
     # duration, against the 8 TB/s HBM peak
     f_us, a_us = float(np.mean(fwd_us)), float(np.mean(adj_us))
     kf, ka = int(stage[-1]["fwd_steps_per_launch"]), int(stage[-1]["adj_steps_per_launch"])
-    copy_gbs = copy_peak()
+    copy_gbs, plain_copy_gbs = copy_peak()
 
     def roofline(adjoint):
         k, kus = (ka, a_us) if adjoint else (kf, f_us)
@@ -495,6 +500,11 @@ def main():
              "hbm_gbs_measured": round(traffic / (kus * 1e-6) / 1e9, 1) if traffic else None,
              "hbm_frac_measured": round(traffic / (kus * 1e-6) / 1e9 / PEAK_HBM_GBS, 4) if traffic else None,
              "copy_peak_gbs": round(copy_gbs, 1),
+             "traffic_over_copy_peak": round(traffic / (kus * 1e-6) / 1e9 / copy_gbs, 3) if traffic else None,
+             "plain_copy_gbs": round(plain_copy_gbs, 1),
+             "copy_model": "copy_peak_gbs: 1 GiB copied by the fastest streaming copy (non-temporal 16-byte loads "
+                           "and stores, four in flight per lane); plain_copy_gbs: the grid-stride float4 copy that "
+                           "calibrates the traffic counters",
              "nominal_bytes_per_launch": int(nominal),
              "nominal_gbs": round(nominal / (kus * 1e-6) / 1e9, 1),
              "nominal_model": "k x the unblocked per-step model (60 B per padded cell-step + snapshot and image "

@@ -326,6 +326,13 @@ int swd_batch_gradient(swd_engine **engines, int n, double **dk_vs, double *kern
  * readings scale the FD kernels' readings to bytes. */
 int swd_calib_copy(long long bytes, int reps, double *ms);
 
+/* Practical HBM ceiling (no reference counterpart): the fastest device copy
+ * this library has measured on MI355X (non-temporal 16-byte loads and
+ * stores, four in flight per lane); copies `bytes` `reps` times, *ms = mean
+ * event-timed duration of one copy.  Read + write bytes over *ms is the rate
+ * a streaming kernel can reach, beside the 8 TB/s specification. */
+int swd_copy_peak(long long bytes, int reps, double *ms);
+
 /*
  * The dispersion path alone, batched over every (shot, side) column of
  * nshots gathers (the per-shot RTr/RTl + LHDispPick of SWD.m:173-185 and

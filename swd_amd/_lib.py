@@ -85,6 +85,7 @@ SIGNATURES = {
     "swd_dispersion_pick": (_I, [_P, _P, _P, _P, _P, _P]),
     "swd_dispersion_adjoint": (_I, [_P, ctypes.c_double, _P, _P]),
     "swd_calib_copy": (_I, [ctypes.c_longlong, _I, _P]),
+    "swd_copy_peak": (_I, [ctypes.c_longlong, _I, _P]),
     "swd_server_create": (_I, [ctypes.c_longlong, _I, _I, _I, _P]),
     "swd_server_destroy": (_I, [_P]),
     "swd_server_submit": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
