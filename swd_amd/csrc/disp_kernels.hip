@@ -690,6 +690,41 @@ __global__ __launch_bounds__(256) void radon_abs_kernel(const float2* MM, int np
 // of equal values whose previous sample is larger and whose run is followed
 // by a larger sample (MATLAB findpeaks defaults; endpoints never count).
 // ---------------------------------------------------------------------------
+// Wave-wide argmax by DPP: row_shr 1, 2, 4, 8 leave each row's (value,
+// index) maximum in its lane 15, row_bcast 15 and 31 carry rows 0-2 into row
+// 3, so lane 63 holds the wave's; a lane whose DPP source is invalid keeps its
+// own pair.  Ties go to the smaller index, NaN never enters.
+template <int CTRL, int RM>
+__device__ __forceinline__ int dpp_keep(int x) { return __builtin_amdgcn_update_dpp(x, x, CTRL, RM, 0xf, false); }
+template <int CTRL, int RM>
+__device__ __forceinline__ float dpp_keep(float x) { return __int_as_float(dpp_keep<CTRL, RM>(__float_as_int(x))); }
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp_keep(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = dpp_keep<CTRL, RM>((int)(b & 0xffffffffLL)), hi = dpp_keep<CTRL, RM>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int CTRL, int RM, typename V>
+__device__ __forceinline__ void argmax_dpp_step(V& best, int& bi)
+{
+    const V ob = dpp_keep<CTRL, RM>(best);
+    const int oi = dpp_keep<CTRL, RM>(bi);
+    if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+}
+// the index of the wave's maximum (uniform)
+template <typename V>
+__device__ __forceinline__ int wave_argmax_index(V best, int bi)
+{
+    argmax_dpp_step<0x111, 0xf>(best, bi);  // row_shr:1
+    argmax_dpp_step<0x112, 0xf>(best, bi);  // row_shr:2
+    argmax_dpp_step<0x114, 0xf>(best, bi);  // row_shr:4
+    argmax_dpp_step<0x118, 0xf>(best, bi);  // row_shr:8
+    argmax_dpp_step<0x142, 0xa>(best, bi);  // row_bcast:15 into rows 1 and 3
+    argmax_dpp_step<0x143, 0xc>(best, bi);  // row_bcast:31 into rows 2 and 3
+    return __builtin_amdgcn_readlane(bi, 63);
+}
+
 // is position p (0-based within v[lo..hi]) a findpeaks location of -v ?
 template <typename V>
 __device__ bool is_local_min(const V* v, int lo, int hi, int p)
@@ -712,7 +747,17 @@ __device__ bool is_local_min(const V* v, int lo, int hi, int p)
 // columns r0..npair-1-r1 (0-based; the field scripts' cr_pre(a:end-b)).
 // V is the spectrum's type: float for the engine and swd_lhdisppick, double
 // for callers that pick on FP64 spectra (swd_lhdisppick_f64).
-template <typename V>
+//
+// The tracking is sequential over the frequency columns, so one wavefront
+// per (shot, side) spends most of a step waiting for its next column.  The
+// columns ahead stream through a ring of kPickRing register buffers
+// (NK values per lane each), issued kPickRing - 1 steps before the column is
+// staged in LDS; every load is unconditional (indices clamped into the
+// spectrum), so the compiler's wait counts are exact and no wait reaches
+// past the column that is needed.
+constexpr int kPickRing = 4;
+
+template <typename V, int NK>
 __global__ __launch_bounds__(64) void pick_kernel(const V* ml, const V* mlr, const DispCol* cols, int np,
                                                   int npair, int lp, float fg, float lfg, double vmin, int r0, int r1,
                                                   const double* cr0, double* cr_pre, double* res, int* status)
@@ -723,26 +768,24 @@ __global__ __launch_bounds__(64) void pick_kernel(const V* ml, const V* mlr, con
     __shared__ int idx[kMaxPair], dvmax[kMaxPair], dvmin[kMaxPair], chz[kMaxPair];
     __shared__ float qj[kMaxPair];
     __shared__ double fidx[kMaxPair];
-    // the frequency column being tracked and the next one, staged in LDS: the
-    // next column's loads are in flight while the current one is searched
+    // the frequency column being tracked and the next one, staged in LDS
     __shared__ V colv[2][kPickMaxNp];
     if (!c.enabled) return;
     const V* base = ml + (size_t)col * npair * np;
     const int a = np;
-    const int nk = (np + 63) >> 6;
-    V pf[kPickMaxNp / 64];
-    auto fetch = [&](int i) {
+    V pf[kPickRing][NK];
+    // column ci (clamped into 0..npair-1) into ring buffer u; the element
+    // index is clamped too, so every lane issues NK loads
+    auto fetch = [&](V* buf, int ci) {
+        const V* cb = base + (size_t)min(max(ci, 0), npair - 1) * np;
 #pragma unroll
-        for (int k = 0; k < kPickMaxNp / 64; ++k) {
-            const int j = k * 64 + lane;
-            if (k < nk) pf[k] = j < np ? base[(size_t)i * np + j] : V(0);
-        }
+        for (int k = 0; k < NK; ++k) buf[k] = cb[min(k * 64 + lane, np - 1)];
     };
-    auto put = [&](int b) {
+    auto put = [&](const V* buf, int b) {
 #pragma unroll
-        for (int k = 0; k < kPickMaxNp / 64; ++k) {
+        for (int k = 0; k < NK; ++k) {
             const int j = k * 64 + lane;
-            if (k < nk && j < np) colv[b][j] = pf[k];
+            if (j < np) colv[b][j] = buf[k];
         }
     };
 
@@ -754,36 +797,9 @@ __global__ __launch_bounds__(64) void pick_kernel(const V* ml, const V* mlr, con
             const V x = v[i];
             if (x == x && (x > best || (x == best && i < bi))) { best = x; bi = i; }
         }
-        for (int off = 32; off > 0; off >>= 1) {
-            const V ob = __shfl_xor(best, off);
-            const int oi = __shfl_xor(bi, off);
-            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-        }
+        bi = wave_argmax_index(best, bi);
         return bi == 0x7fffffff ? 1 : bi - lo + 1;
     };
-    // first local minimum at or after position 0 of v[lo..hi] (1-based location), 0 if none
-    auto first_min = [&](const V* v, int lo, int hi) -> int {
-        const int n = hi - lo + 1;
-        for (int p0 = 0; p0 < n; p0 += 64) {
-            const int p = p0 + lane;
-            const bool f = p < n && is_local_min(v, lo, hi, p);
-            const unsigned long long m = __ballot(f);
-            if (m) return p0 + __ffsll((long long)m);   // 1-based
-        }
-        return 0;
-    };
-    // last local minimum of v[lo..hi] (1-based location), 0 if none
-    auto last_min = [&](const V* v, int lo, int hi) -> int {
-        const int n = hi - lo + 1;
-        for (int p0 = ((n - 1) / 64) * 64; p0 >= 0; p0 -= 64) {
-            const int p = p0 + lane;
-            const bool f = p < n && is_local_min(v, lo, hi, p);
-            const unsigned long long m = __ballot(f);
-            if (m) return p0 + 64 - __clzll((long long)m);  // 1-based
-        }
-        return 0;
-    };
-
     if (lp == 0) {
         // column maxima of |mm| (NaN ignored), then the first column holding the largest
         const V* u = mlr + (size_t)col * npair * np;
@@ -799,73 +815,167 @@ __global__ __launch_bounds__(64) void pick_kernel(const V* ml, const V* mlr, con
     }
     const int L1 = lp - 1;  // 0-based index of lp
     if (lane == 0) for (int i = 0; i < npair; ++i) { chz[i] = 1; qj[i] = NAN; dvmax[i] = 0; dvmin[i] = 0; }
-    fetch(L1);
-    put(0);
+    fetch(pf[0], L1);
+    put(pf[0], 0);
     __syncthreads();
-    int cur = argmax(colv[0], 0, a - 1);
+    const int cur = argmax(colv[0], 0, a - 1);
     if (lane == 0) idx[L1] = cur;
-    if (L1 >= 1) {
-        fetch(L1 - 1);
-        put(1);
-    }
     __syncthreads();
-    // left of lp (column i in colv[b])
-    int b = 1;
-    for (int i = L1 - 1; i >= 0; --i) {
-        if (i >= 1) fetch(i - 1);
-        const V* v = colv[b];
-        const int ip = idx[i + 1];
-        if (ip == a) {
-            if (lane == 0) { idx[i] = a; dvmax[i] = dvmax[i + 1]; dvmin[i] = dvmin[i + 1]; }
-        } else {
-            int dmax, dmin;
-            if (ip > a - 3) dmax = a;
-            else { const int f = first_min(v, ip - 1, a - 1); dmax = f ? ip + f : a; }
-            if (ip < 3) dmin = 1;
-            else { const int f = last_min(v, 0, ip - 1); dmin = f ? f : 1; }
-            const float q = (float)(dmax - dmin);
-            const float qprev = qj[i + 1];
-            if (q > qprev * lfg) {
-                if (lane == 0) { chz[i] = 0; dvmax[i] = dvmax[i + 1]; dvmin[i] = dvmin[i + 1]; qj[i] = (float)(dvmax[i + 1] - dvmin[i + 1]); idx[i] = ip; }
-            } else {
-                const int newidx = dmin + argmax(v, dmin - 1, dmax - 1) - 1;
-                if (lane == 0) { idx[i] = newidx; dvmax[i] = dmax; dvmin[i] = dmin; qj[i] = q; }
+
+    // lane k's 64-bit word (k uniform), through scalar reads
+    auto lane_word = [](unsigned long long m, int k) -> unsigned long long {
+        const unsigned lo = __builtin_amdgcn_readlane((unsigned)(m & 0xffffffffull), k);
+        const unsigned hi = __builtin_amdgcn_readlane((unsigned)(m >> 32), k);
+        return ((unsigned long long)hi << 32) | lo;
+    };
+    // The local minima of the whole column (findpeaks of -v over v[0..a-1]):
+    // lane k < NK holds the bits of positions 64k .. 64k+63.  The windows of
+    // a step are read off this mask:
+    //  - first_min(v, ip-1, a-1) finds the first position i >= ip that is a
+    //    minimum of the column (position ip-1 itself is excluded, and the
+    //    window ends where the column does, so nothing else differs);
+    //  - last_min(v, 0, ip-1) finds the last minimum i <= ip-1 whose run of
+    //    equal values ends before ip-1.  Only the last minimum of the column
+    //    at or below ip-1 can have a run crossing ip-1 (an earlier one ends
+    //    below the next minimum), so at most one candidate is skipped.
+    // Built without branches: each lane reads the left, centre and right
+    // values of its NK positions (independent LDS reads, clamped indices) and
+    // tests for a strict minimum, which is the findpeaks rule wherever the
+    // right neighbour differs.  A position whose right neighbour is equal
+    // starts a run of equal values; those (rare in spectra) are decided by
+    // the full rule.
+    auto min_mask = [&](const V* v) -> unsigned long long {
+        unsigned long long mine = 0, runs = 0;
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+            const int p = k * 64 + lane;
+            const int pc = min(p, a - 1);
+            const V y = v[pc], yl = v[max(pc - 1, 0)], yr = v[min(pc + 1, a - 1)];
+            const bool in = p >= 1 && p <= a - 2;  // the first and last positions are never minima
+            const unsigned long long m = __ballot(in && yl > y && yr > y);
+            const unsigned long long r = __ballot(in && yl > y && yr == y);
+            if (lane == k) { mine = m; runs = r; }
+        }
+        if (__ballot(runs != 0ull)) {
+#pragma unroll
+            for (int k = 0; k < NK; ++k) {
+                const unsigned long long rk = lane_word(runs, k);
+                if (rk) {
+                    const int p = k * 64 + lane;
+                    const unsigned long long m = __ballot(((rk >> lane) & 1ull) && is_local_min(v, 0, a - 1, p));
+                    if (lane == k) mine |= m;
+                }
             }
         }
-        __syncthreads();
-        if (i >= 1) put(b ^ 1);
-        __syncthreads();
-        b ^= 1;
-    }
-    // right of lp
-    if (L1 + 1 < npair) {
-        fetch(L1 + 1);
-        put(0);
-    }
-    __syncthreads();
-    b = 0;
-    for (int i = L1 + 1; i < npair; ++i) {
-        if (i + 1 < npair) fetch(i + 1);
-        const V* v = colv[b];
-        const int ip = idx[i - 1];
+        return mine;
+    };
+    // first set position >= s, -1 if none
+    auto first_from = [&](unsigned long long mine, int s) -> int {
+        const int w = s >> 6;
+        unsigned long long m = (lane < w || lane >= NK) ? 0ull : mine;
+        if (lane == w) m &= ~0ull << (s & 63);
+        const unsigned long long nz = __ballot(m != 0ull);
+        if (!nz) return -1;
+        const int k = __ffsll((long long)nz) - 1;
+        const unsigned long long mk = lane_word(m, k);
+        return k * 64 + __ffsll((long long)mk) - 1;
+    };
+    // last set position <= t, -1 if none
+    auto last_upto = [&](unsigned long long mine, int t) -> int {
+        if (t < 0) return -1;
+        const int w = t >> 6;
+        unsigned long long m = (lane > w || lane >= NK) ? 0ull : mine;
+        if (lane == w && (t & 63) != 63) m &= (1ull << ((t & 63) + 1)) - 1ull;
+        const unsigned long long nz = __ballot(m != 0ull);
+        if (!nz) return -1;
+        const int k = 63 - __clzll((long long)nz);
+        const unsigned long long mk = lane_word(m, k);
+        return k * 64 + 63 - __clzll((long long)mk);
+    };
+
+    // The tracking state of the last column (its idx, qj, dvmax, dvmin),
+    // carried in registers from step to step; lane 0 also records each
+    // column's values in LDS for the interpolation at the end.
+    struct Track {
+        int ip;
+        float q;
+        int dmax, dmin;
+    };
+    // one tracking step at column i from its neighbour's state t
+    // (left sweep: dir = -1, width jumps above lfg rejected; right: dir = +1, fg)
+    auto step = [&](const V* v, int i, int dir, const Track t) __attribute__((always_inline)) -> Track {
+        const int ip = t.ip;
+        if (dir < 0 && ip == a) {
+            // carried over; qj[i] stays NaN and chz[i] stays 1
+            if (lane == 0) { idx[i] = a; dvmax[i] = t.dmax; dvmin[i] = t.dmin; }
+            return Track{a, NAN, t.dmax, t.dmin};
+        }
+        const unsigned long long mine = (ip > a - 3 && ip < 3) ? 0ull : min_mask(v);
         int dmax, dmin;
         if (ip > a - 3) dmax = a;
-        else { const int f = first_min(v, ip - 1, a - 1); dmax = f ? ip + f : a; }
-        if (ip < 3) dmin = 1;
-        else { const int f = last_min(v, 0, ip - 1); dmin = f ? f : 1; }
-        const float q = (float)(dmax - dmin);
-        const float qprev = qj[i - 1];
-        if (q > fg * qprev) {
-            if (lane == 0) { chz[i] = 0; dvmax[i] = dvmax[i - 1]; dvmin[i] = dvmin[i - 1]; qj[i] = (float)(dvmax[i - 1] - dvmin[i - 1]); idx[i] = ip; }
-        } else {
-            const int newidx = dmin + argmax(v, dmin - 1, dmax - 1) - 1;
-            if (lane == 0) { idx[i] = newidx; dvmax[i] = dmax; dvmin[i] = dmin; qj[i] = q; }
+        else {
+            // first_min(v, ip - 1, a - 1): 1-based location in the window
+            const int m = first_from(mine, ip);
+            dmax = m >= 0 ? ip + (m - (ip - 1) + 1) : a;
         }
+        if (ip < 3) dmin = 1;
+        else {
+            // last_min(v, 0, ip - 1)
+            const int hi = ip - 1;
+            int m = last_upto(mine, hi);
+            if (m >= 0) {
+                const V y = v[m];
+                int e = m;
+                while (e + 1 <= hi && v[e + 1] == y) ++e;
+                if (e + 1 > hi) m = last_upto(mine, m - 1);
+            }
+            dmin = m >= 0 ? m + 1 : 1;
+        }
+        const float q = (float)(dmax - dmin);
+        const float qprev = t.q;
+        if (dir < 0 ? q > qprev * lfg : q > fg * qprev) {
+            const float qc = (float)(t.dmax - t.dmin);
+            if (lane == 0) { chz[i] = 0; dvmax[i] = t.dmax; dvmin[i] = t.dmin; qj[i] = qc; idx[i] = ip; }
+            return Track{ip, qc, t.dmax, t.dmin};
+        }
+        const int newidx = dmin + argmax(v, dmin - 1, dmax - 1) - 1;
+        if (lane == 0) { idx[i] = newidx; dvmax[i] = dmax; dvmin[i] = dmin; qj[i] = q; }
+        return Track{newidx, q, dmax, dmin};
+    };
+    // columns c0, c0 + dir, ... (n of them): column j is fetched into ring
+    // buffer j % kPickRing at step j - kPickRing (or before the sweep) and
+    // staged in LDS at the end of step j - 1
+    auto sweep = [&](int c0, int dir, int n) __attribute__((always_inline)) {
+        if (n <= 0) return;
+        // the start column: idx = the argmax, dvmax = dvmin = 0, qj = NaN
+        Track t{cur, NAN, 0, 0};
+#pragma unroll
+        for (int u = 0; u < kPickRing; ++u) fetch(pf[u], c0 + dir * min(u, n - 1));
+        put(pf[0], 0);
         __syncthreads();
-        if (i + 1 < npair) put(b ^ 1);
-        __syncthreads();
-        b ^= 1;
-    }
+        // Every sub-step stages and fetches a column, also past the end of the
+        // sweep (clamped, unused), so that each issues the same loads: the
+        // compiler then waits only for the column being staged, which was
+        // requested kPickRing - 1 steps earlier.  A tracking step guarded by
+        // the same condition as its loads would let it assume the shortest
+        // path and wait for the loads just issued.
+        int b = 0;
+        for (int j0 = 0; j0 < n; j0 += kPickRing) {
+#pragma unroll
+            for (int u = 0; u < kPickRing; ++u) {
+                const int j = j0 + u;
+                if (j < n) t = step(colv[b], c0 + dir * j, dir, t);
+                __syncthreads();
+                // column j + 1 into LDS; column j + kPickRing into the freed buffer
+                put(pf[(u + 1) % kPickRing], b ^ 1);
+                fetch(pf[u], c0 + dir * min(j + kPickRing, n - 1));
+                __syncthreads();
+                b ^= 1;
+            }
+        }
+    };
+    sweep(L1 - 1, -1, L1);               // left of lp
+    sweep(L1 + 1, 1, npair - 1 - L1);   // right of lp
     // interp1(que, idx(que), 1:b, 'linear') then nearest extrapolation
     if (lane == 0) {
         int nq = 0, qfirst = -1, qlast = -1;
@@ -1096,7 +1206,16 @@ static int pick_launch(hipStream_t st, const V* ml, const V* mlr, const DispCol*
                        double* res, int* status)
 {
     if (lp == 0 && !mlr) { set_error("an automatic pick start needs |mm|"); return kErrArg; }
-    pick_kernel<V><<<ncol, 64, 0, st>>>(ml, mlr, cols, np, npair, lp, fg, lfg, vmin, r0, r1, cr0, cr_pre, res, status);
+    if (np < 1 || np > kPickMaxNp || npair < 1 || npair > kMaxPair) {
+        set_error("pick of %d velocities x %d columns outside 1..%d x 1..%d", np, npair, kPickMaxNp, kMaxPair);
+        return kErrArg;
+    }
+    if (np <= 12 * 64)
+        pick_kernel<V, 12><<<ncol, 64, 0, st>>>(ml, mlr, cols, np, npair, lp, fg, lfg, vmin, r0, r1, cr0, cr_pre, res,
+                                                status);
+    else
+        pick_kernel<V, kPickMaxNp / 64><<<ncol, 64, 0, st>>>(ml, mlr, cols, np, npair, lp, fg, lfg, vmin, r0, r1, cr0,
+                                                             cr_pre, res, status);
     SWD_HIP(hipGetLastError());
     return kOk;
 }

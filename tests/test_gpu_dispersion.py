@@ -129,3 +129,31 @@ def test_fp64_spectra_are_picked_in_fp64(gather, side):
         ro, co = O.LHDispPick(ml_o, 100, cr0, method, 10)
         np.testing.assert_array_equal(cg, co)
         assert rg == pytest.approx(ro, rel=1e-15)
+
+
+@pytest.mark.parametrize("np_", [97, 701, 768, 769, 1500, 2048])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_pick_random_ridges_plateaus_and_nans(np_, dtype):
+    """pickFDC1 through LHDispPick on spectra built to hit every branch of the
+    pick kernel's local-minimum search: several drifting ridges (width jumps
+    and rejections), values quantised to 1/64 (runs of equal values, where
+    findpeaks' run rule decides), a few NaNs, and velocity counts on both
+    sides of the kernel's 768-velocity register ring.  Picks must equal the
+    oracle's exactly (same spectrum into both)."""
+    r = np.random.default_rng(np_ * 7 + (1 if dtype == np.float64 else 0))
+    npair = 61
+    j = np.arange(1, np_ + 1)[:, None]
+    ml = 0.02 * r.random((np_, npair))
+    for _ in range(4):
+        c0, drift, w = r.uniform(0.1, 0.9) * np_, r.uniform(-0.3, 0.3) * np_ / npair, r.uniform(0.01, 0.05) * np_
+        centres = c0 + drift * np.arange(npair)
+        ml += r.uniform(0.3, 1.0) * np.exp(-0.5 * ((j - centres[None, :]) / w) ** 2)
+    ml = np.round(ml * 64) / 64                     # plateaus of equal values
+    ml[r.integers(0, np_, 5), r.integers(0, npair, 5)] = np.nan
+    ml = ml.astype(dtype)
+    for ini in (1, 30, npair):
+        cr0 = np.full(npair, 0.5 * np_)
+        res_g, cr_g = Dp.LHDispPick(ml, 100, cr0, 1, ini)
+        res_o, cr_o = O.LHDispPick(ml.astype(np.float64), 100, cr0, 1, ini)
+        np.testing.assert_array_equal(cr_g, cr_o)
+        assert res_g == pytest.approx(res_o, rel=1e-12)
