@@ -41,125 +41,89 @@ __global__ void dft_table_kernel(float* W, int nt_eff, int npair, int c0, int cc
 // ---------------------------------------------------------------------------
 // Strided batched FP32 GEMM on MFMA 16x16x4:
 //   C(b, m, n) = alpha * sum_k A(b, m, k) B(b, k, n)
-// 64x64 tile per 256-thread block, 4 wavefronts x (32x32).  Two kernels
-// with the same MFMA chain (plain ascending k), hence bitwise equal results:
-//   gemm_f32_kernel: K tiles of 16 through one LDS buffer (short K);
-//   gemm_f32_db:     K tiles of 32 double-buffered in LDS, the next tile's
-//                    global loads in flight during the current tile's MFMAs,
-//                    one barrier per tile (long K: the DFT at the bins).
+// BM x BN tile per 256-thread block, 2 x 2 wavefronts of (BM/2) x (BN/2)
+// (BM/32 x BN/32 MFMA tiles each); K tiles of BK double-buffered in LDS, the
+// next tile's global loads in flight during the current tile's MFMAs, one
+// barrier per tile.  Every shape runs the same MFMA chain per output element
+// (ascending k, four at a time), so tile shapes and K tiles give bitwise
+// equal results.  Measured on the production shapes (60 shots): the inverse
+// DFT (M 6000, N 600, K 302) at 128 x 128, BK 16: 1.37 ms in the gradient,
+// against 1.99 ms for the 64 x 64 single-buffer kernel it replaced; the DFT
+// at the bins (M 302, N 600, K 6000) stays at 64 x 64, BK 32 (1.70 ms;
+// 64 x 128 took 1.73 ms there, though a standalone run favoured it).
 // ---------------------------------------------------------------------------
 constexpr int GBM = 64, GBN = 64, GBK = 16;
 
-__global__ __launch_bounds__(256) void gemm_f32_kernel(GemmArgs g)
-{
-    __shared__ float As[GBK][GBM + 4];
-    __shared__ float Bs[GBK][GBN + 4];
-    const int b = blockIdx.z;
-    const int m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const float* A = g.A + (size_t)b * g.sAb;
-    const float* B = g.B + (size_t)b * g.sBb;
-    const bool a_mfast = g.sAm == 1;
-    const bool b_nfast = g.sBn == 1;
-    const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
-    v4f acc[2][2];
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < g.K; k0 += GBK) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int e = tid + 256 * q;
-            int mm, kk;
-            if (a_mfast) { mm = e % GBM; kk = e / GBM; } else { kk = e % GBK; mm = e / GBK; }
-            const int gm = m0 + mm, gk = k0 + kk;
-            As[kk][mm] = (gm < g.M && gk < g.K) ? A[(size_t)gm * g.sAm + (size_t)gk * g.sAk] : 0.f;
-            int nn;
-            if (b_nfast) { nn = e % GBN; kk = e / GBN; } else { kk = e % GBK; nn = e / GBK; }
-            const int gn = n0 + nn, gk2 = k0 + kk;
-            Bs[kk][nn] = (gn < g.N && gk2 < g.K) ? B[(size_t)gk2 * g.sBk + (size_t)gn * g.sBn] : 0.f;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < GBK; kk += 4) {
-            const int kr = kk + (lane >> 4);
-            const float a0 = As[kr][wm + (lane & 15)];
-            const float a1 = As[kr][wm + 16 + (lane & 15)];
-            const float b0 = Bs[kr][wn + (lane & 15)];
-            const float b1 = Bs[kr][wn + 16 + (lane & 15)];
-            acc[0][0] = mfma16x16x4(a0, b0, acc[0][0]);
-            acc[0][1] = mfma16x16x4(a0, b1, acc[0][1]);
-            acc[1][0] = mfma16x16x4(a1, b0, acc[1][0]);
-            acc[1][1] = mfma16x16x4(a1, b1, acc[1][1]);
-        }
-        __syncthreads();
-    }
-    float* C = g.C + (size_t)b * g.sCb;
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int gm = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
-                const int gn = n0 + wn + j * 16 + (lane & 15);
-                if (gm < g.M && gn < g.N) C[(size_t)gm * g.sCm + (size_t)gn * g.sCn] = g.alpha * acc[i][j][r];
-            }
-}
+// LDS layout of an operand tile, chosen at compile time by the operand's
+// global layout so that the coalesced stores and the fragment reads (16 rows
+// m at k .. k+3) are free of bank conflicts:
+//   m (or n) contiguous in memory: [k][m], rows R + 16 floats apart (== 16 mod 64);
+//   k contiguous in memory:        [m][k], rows BK + 2 floats apart.
+template <int R, int BK, bool MF>
+struct GemmLds {
+    static constexpr int S = MF ? R + 16 : BK + 2;
+    static constexpr int SIZE = MF ? BK * S : R * S;
+    static __device__ __forceinline__ int off(int mm, int kk) { return MF ? kk * S + mm : mm * S + kk; }
+};
 
-// LDS layout of a double-buffered operand tile, chosen at compile time by the
-// operand's global layout so that the coalesced stores and the fragment reads
-// (16 rows m at k and at k+1) are both free of bank conflicts:
-//   m (or n) contiguous in memory: [k][m], rows 80 floats apart (80 = 16 mod 32);
-//   k contiguous in memory:        [m][k], rows 34 floats apart (34 = 2 mod 32).
-constexpr int GDK = 32, GDS = 80, GDT = 34, GDBUF = GDK * GDS;  // GDBUF >= 64 * GDT
-template <bool MF>
-__device__ __forceinline__ int lds_off(int mm, int kk) { return MF ? kk * GDS + mm : mm * GDT + kk; }
-
-template <bool AMF, bool BNF>
+template <int BM, int BN, int BK, bool AMF, bool BNF>
 __global__ __launch_bounds__(256) void gemm_f32_db(GemmArgs g)
 {
-    __shared__ float As[2][GDBUF];
-    __shared__ float Bs[2][GDBUF];
+    using LA = GemmLds<BM, BK, AMF>;
+    using LB = GemmLds<BN, BK, BNF>;
+    constexpr int TM = BM / 32, TN = BN / 32, NA = BM * BK / 256, NB = BN * BK / 256;
+    __shared__ float As[2][LA::SIZE];
+    __shared__ float Bs[2][LB::SIZE];
     const int b = blockIdx.z;
-    const int m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const float* A = g.A + (size_t)b * g.sAb;
     const float* B = g.B + (size_t)b * g.sBb;
-    const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
-    v4f acc[2][2];
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-    // element q of this thread's share of a 64 x 32 tile: (row of m or n, k);
+    const int wm = (w & 1) * (BM / 2), wn = (w >> 1) * (BN / 2);
+    v4f acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    // element q of this thread's share of a tile: (row of m or n, k);
     // consecutive threads walk the operand's contiguous index
     auto a_at = [&](int q, int& mm, int& kk) {
         const int e = tid + 256 * q;
-        if (AMF) { mm = e % GBM; kk = e / GBM; } else { kk = e % GDK; mm = e / GDK; }
+        if (AMF) { mm = e % BM; kk = e / BM; } else { kk = e % BK; mm = e / BK; }
     };
     auto b_at = [&](int q, int& nn, int& kk) {
         const int e = tid + 256 * q;
-        if (BNF) { nn = e % GBN; kk = e / GBN; } else { kk = e % GDK; nn = e / GDK; }
+        if (BNF) { nn = e % BN; kk = e / BN; } else { kk = e % BK; nn = e / BK; }
     };
-    float ra[8], rb[8];
+    float ra[NA], rb[NB];
     auto gload = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            int mm, kk, nn, kb;
+        for (int q = 0; q < NA; ++q) {
+            int mm, kk;
             a_at(q, mm, kk);
             const int gm = m0 + mm, gk = k0 + kk;
             ra[q] = (gm < g.M && gk < g.K) ? A[(size_t)gm * g.sAm + (size_t)gk * g.sAk] : 0.f;
-            b_at(q, nn, kb);
-            const int gn = n0 + nn, gk2 = k0 + kb;
-            rb[q] = (gn < g.N && gk2 < g.K) ? B[(size_t)gk2 * g.sBk + (size_t)gn * g.sBn] : 0.f;
+        }
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            int nn, kk;
+            b_at(q, nn, kk);
+            const int gn = n0 + nn, gk = k0 + kk;
+            rb[q] = (gn < g.N && gk < g.K) ? B[(size_t)gk * g.sBk + (size_t)gn * g.sBn] : 0.f;
         }
     };
     auto lstore = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            int mm, kk, nn, kb;
+        for (int q = 0; q < NA; ++q) {
+            int mm, kk;
             a_at(q, mm, kk);
-            As[buf][lds_off<AMF>(mm, kk)] = ra[q];
-            b_at(q, nn, kb);
-            Bs[buf][lds_off<BNF>(nn, kb)] = rb[q];
+            As[buf][LA::off(mm, kk)] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            int nn, kk;
+            b_at(q, nn, kk);
+            Bs[buf][LB::off(nn, kk)] = rb[q];
         }
     };
     gload(0);
@@ -167,20 +131,21 @@ __global__ __launch_bounds__(256) void gemm_f32_db(GemmArgs g)
     __syncthreads();
     int cur = 0;
     const int l16 = lane & 15, lk = lane >> 4;
-    for (int k0 = 0; k0 < g.K; k0 += GDK) {
-        const bool more = k0 + GDK < g.K;
-        if (more) gload(k0 + GDK);
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+        const bool more = k0 + BK < g.K;
+        if (more) gload(k0 + BK);
 #pragma unroll
-        for (int kk = 0; kk < GDK; kk += 4) {
+        for (int kk = 0; kk < BK; kk += 4) {
             const int kr = kk + lk;
-            const float a0 = As[cur][lds_off<AMF>(wm + l16, kr)];
-            const float a1 = As[cur][lds_off<AMF>(wm + 16 + l16, kr)];
-            const float b0 = Bs[cur][lds_off<BNF>(wn + l16, kr)];
-            const float b1 = Bs[cur][lds_off<BNF>(wn + 16 + l16, kr)];
-            acc[0][0] = mfma16x16x4(a0, b0, acc[0][0]);
-            acc[0][1] = mfma16x16x4(a0, b1, acc[0][1]);
-            acc[1][0] = mfma16x16x4(a1, b0, acc[1][0]);
-            acc[1][1] = mfma16x16x4(a1, b1, acc[1][1]);
+            float a[TM], bb[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) a[i] = As[cur][LA::off(wm + i * 16 + l16, kr)];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) bb[j] = Bs[cur][LB::off(wn + j * 16 + l16, kr)];
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j) acc[i][j] = mfma16x16x4(a[i], bb[j], acc[i][j]);
         }
         if (more) lstore(cur ^ 1);
         __syncthreads();
@@ -188,15 +153,25 @@ __global__ __launch_bounds__(256) void gemm_f32_db(GemmArgs g)
     }
     float* C = g.C + (size_t)b * g.sCb;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < TN; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int gm = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
-                const int gn = n0 + wn + j * 16 + (lane & 15);
+                const int gn = n0 + wn + j * 16 + l16;
                 if (gm < g.M && gn < g.N) C[(size_t)gm * g.sCm + (size_t)gn * g.sCn] = g.alpha * acc[i][j][r];
             }
+}
+
+template <int BM, int BN, int BK>
+static void launch_gemm_db(hipStream_t st, const GemmArgs& g, int batch)
+{
+    const dim3 grid(div_up(g.N, BN), div_up(g.M, BM), batch);
+    if (g.sAm == 1 && g.sBn == 1) gemm_f32_db<BM, BN, BK, true, true><<<grid, 256, 0, st>>>(g);
+    else if (g.sAm == 1) gemm_f32_db<BM, BN, BK, true, false><<<grid, 256, 0, st>>>(g);
+    else if (g.sBn == 1) gemm_f32_db<BM, BN, BK, false, true><<<grid, 256, 0, st>>>(g);
+    else gemm_f32_db<BM, BN, BK, false, false><<<grid, 256, 0, st>>>(g);
 }
 
 // ---------------------------------------------------------------------------
@@ -1122,13 +1097,14 @@ int launch_dft_table(hipStream_t st, float* W, int nt_eff, int npair, int c0, in
 int launch_gemm_f32(hipStream_t st, const GemmArgs& g, int batch)
 {
     if (g.M <= 0 || g.N <= 0 || batch <= 0) return kOk;
-    const dim3 grid(div_up(g.N, GBN), div_up(g.M, GBM), batch);
-    if (radon_precision() == 1) gemm_bf16x3_kernel<<<grid, 256, 0, st>>>(g);
-    else if (g.K < 512) gemm_f32_kernel<<<grid, 256, 0, st>>>(g);
-    else if (g.sAm == 1 && g.sBn == 1) gemm_f32_db<true, true><<<grid, 256, 0, st>>>(g);
-    else if (g.sAm == 1) gemm_f32_db<true, false><<<grid, 256, 0, st>>>(g);
-    else if (g.sBn == 1) gemm_f32_db<false, true><<<grid, 256, 0, st>>>(g);
-    else gemm_f32_db<false, false><<<grid, 256, 0, st>>>(g);
+    if (radon_precision() == 1) {
+        const dim3 grid(div_up(g.N, GBN), div_up(g.M, GBM), batch);
+        gemm_bf16x3_kernel<<<grid, 256, 0, st>>>(g);
+    } else if (g.K < 512) {
+        launch_gemm_db<128, 128, 16>(st, g, batch);   // short K: the inverse DFT
+    } else {
+        launch_gemm_db<64, 64, 32>(st, g, batch);     // long K: the DFT at the bins
+    }
     SWD_HIP(hipGetLastError());
     return kOk;
 }
