@@ -179,11 +179,11 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_fwd_march(
                 const size_t plane = (size_t)J.nz * J.nx;
                 const size_t sb = (size_t)(it / J.in_wf) * J.nsf * plane;
                 const size_t so = sb + (size_t)(r - J.pt - 1) * J.nx + (x - J.nbc);
-                if (J.nsf == 3) {   // [row][fux, bwz, fuz + bwx][x]
-                    const size_t sr = sb + (size_t)(r - J.pt - 1) * 3 * J.nx + (x - J.nbc);
+                if (J.nsf == 3) {   // [row][x][fux, bwz, fuz + bwx]
+                    const size_t sr = sb + ((size_t)(r - J.pt - 1) * J.nx + (x - J.nbc)) * 3;
                     stream_st(J.snap + sr, fux);
-                    stream_st(J.snap + sr + J.nx, bwz);
-                    stream_st(J.snap + sr + 2 * J.nx, fuz + bwx);
+                    stream_st(J.snap + sr + 1, bwz);
+                    stream_st(J.snap + sr + 2, fuz + bwx);
                 } else {
                     stream_st(J.snap + so, fux);
                     stream_st(J.snap + so + plane, fuz);
@@ -317,8 +317,8 @@ __global__ __launch_bounds__(kMarchLanes* kMarchWaves) void fd_adj_march(
                 const T* sn = sb + io;
                 double Fux, Fwz, Fsh;
                 if (J.nsf == 3) {
-                    const T* sc = sb + (size_t)(r - J.pt - 1) * 3 * J.nx + (x - J.nbc);
-                    Fux = stream_ld(sc); Fwz = stream_ld(sc + J.nx); Fsh = stream_ld(sc + 2 * J.nx);
+                    const T* sc = sb + ((size_t)(r - J.pt - 1) * J.nx + (x - J.nbc)) * 3;
+                    Fux = stream_ld(sc); Fwz = stream_ld(sc + 1); Fsh = stream_ld(sc + 2);
                 } else {
                     Fux = stream_ld(sn); Fwz = stream_ld(sn + 3 * plane);
                     Fsh = (double)(stream_ld(sn + plane) + stream_ld(sn + 2 * plane));

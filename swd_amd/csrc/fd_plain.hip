@@ -134,7 +134,7 @@ __global__ void plain_snapshot(const T* f, int pitch, int pt, int nbc, int nz, i
     if (ix >= nx || iz >= nz) return;
     const size_t plane = (size_t)nz * nx;
     const T v = f[(size_t)(pt + 1 + iz) * pitch + nbc + ix];
-    if (nsf == 3) snap[(size_t)slot * 3 * plane + ((size_t)iz * 3 + fi) * nx + ix] = v;   // [row][3][x]
+    if (nsf == 3) snap[(size_t)slot * 3 * plane + ((size_t)iz * nx + ix) * 3 + fi] = v;   // [row][x][3]
     else snap[((size_t)slot * nsf + fi) * plane + (size_t)iz * nx + ix] = v;
 }
 
@@ -148,7 +148,7 @@ __global__ void plain_snapshot_sum(const T* fuz, const T* bwx, int pitch, int pt
     if (ix >= nx || iz >= nz) return;
     const size_t plane = (size_t)nz * nx;
     const size_t o = (size_t)(pt + 1 + iz) * pitch + nbc + ix;
-    snap[(size_t)slot * 3 * plane + ((size_t)iz * 3 + 2) * nx + ix] = fuz[o] + bwx[o];
+    snap[(size_t)slot * 3 * plane + ((size_t)iz * nx + ix) * 3 + 2] = fuz[o] + bwx[o];
 }
 
 // adjoint velocity statements
@@ -242,9 +242,9 @@ __global__ void plain_imaging(int which, const T* xx, const T* zz, const T* xz, 
     const size_t plane = (size_t)nz * nx;
     const size_t io = (size_t)iz * nx + ix;
     const T* sn = snap + (size_t)slot * nsf * plane + io;
-    const T* sc = snap + (size_t)slot * 3 * plane + (size_t)iz * 3 * nx + ix;
+    const T* sc = snap + (size_t)slot * 3 * plane + ((size_t)iz * nx + ix) * 3;
     double Fux, Fwz, Fsh;
-    if (nsf == 3) { Fux = sc[0]; Fwz = sc[nx]; Fsh = sc[2 * nx]; }
+    if (nsf == 3) { Fux = sc[0]; Fwz = sc[1]; Fsh = sc[2]; }
     else { Fux = sn[0]; Fwz = sn[3 * plane]; Fsh = (double)(sn[plane] + sn[2 * plane]); }
     const size_t o = (size_t)(pt + 1 + iz) * pitch + nbc + ix;
     const double X = xx[o], Z = zz[o], XZ = xz[o];

@@ -55,7 +55,7 @@ public:
     DevBuf state;     // nshots x 10 planes float
     DevBuf strain;    // 4 planes (plain path scratch)
     DevBuf traces;    // [shot][t][g] float
-    DevBuf snaps;     // [shot][nrec][nz][3][nx] float (fux, bwz, fuz + bwx of each row)
+    DevBuf snaps;     // [shot][nrec][nz][nx][3] float (fux, bwz, fuz + bwx of each cell)
     DevBuf adjsrc;    // [shot][t][g] float
     DevBuf imgs;      // [shot][3][nz][nx] double
     DevBuf imgp;      // [shot][nz][nx][3] float, partial image sums of the fused kernels

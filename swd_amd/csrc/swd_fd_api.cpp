@@ -322,7 +322,7 @@ static int e2drtm_impl(const T* iv, const T* const coef_in[7], const T* adj, con
         SWD_HIP(hipStreamSynchronize(st.s));
     }
     {
-        // the engine's snapshot layout [slot][row][fux, bwz, fuz + bwx][x]: the
+        // the engine's snapshot layout [slot][row][x][fux, bwz, fuz + bwx]: the
         // imaging condition reads only these three (cpp:275-279), and every
         // kernel forms fuz + bwx in T, so the sums are the same as from planes
         std::vector<T> h((size_t)nrec * 3 * splane);
@@ -333,9 +333,9 @@ static int e2drtm_impl(const T* iv, const T* const coef_in[7], const T* adj, con
                 T* row = h.data() + ((size_t)k * d.nz + iz) * 3 * d.nx;
                 for (int ix = 0; ix < d.nx; ++ix) {
                     const size_t c = (size_t)ix * d.nz + iz;
-                    row[ix] = fux[c];
-                    row[d.nx + ix] = bwz[c];
-                    row[2 * d.nx + ix] = fuz[c] + bwx[c];
+                    row[3 * ix] = fux[c];
+                    row[3 * ix + 1] = bwz[c];
+                    row[3 * ix + 2] = fuz[c] + bwx[c];
                 }
             }
         }

@@ -63,9 +63,9 @@ struct FdJob {
     // forward receivers [nt][ng] (g fastest); trace_u may be null
     T* trace_w;
     T* trace_u;
-    // snapshots: nsf = 3, [nt/in_wf][nz][3][nx] (fux, bwz, fuz+bwx, the three
-    // fields of a row back to back); nsf = 4, [nt/in_wf][4][nz][nx] planes
-    // (fux, fuz, bwx, bwz)
+    // snapshots: nsf = 3, [nt/in_wf][nz][nx][3] (fux, bwz, fuz+bwx of a cell
+    // side by side: one 12-byte access per lane); nsf = 4, [nt/in_wf][4][nz][nx]
+    // planes (fux, fuz, bwx, bwz)
     T* snap;
     // adjoint images [3][nz][nx] (cl_img, cm_img, illum), accumulated in double
     double* img;
