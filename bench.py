@@ -207,7 +207,13 @@ def uneven_leg(steps=2, graph=True):
     for e in engs:
         i = e.info()
         cell_steps += e.nshots * i["nzbc"] * i["nxbc"] * (2 * e.problem["nt"] - 1)
-    for e in engs:                         # capture each engine's own graphs before timing
+    # one model at a time, each with the launch shape the autotuner picks for
+    # it alone (the batch's shape can be far from it: 89-row adjoint bands
+    # made the same gradients 2.2x slower), graphs captured before timing
+    seq_shapes = []
+    for e in engs:
+        t = e.autotune()
+        seq_shapes.append([list(t["fwd"]), list(t["adj"])])
         e.gradient()
     t0 = time.perf_counter()
     alone = [e.gradient()["dk_vs"] for e in engs]
@@ -218,6 +224,7 @@ def uneven_leg(steps=2, graph=True):
     return {"value": round(cell_steps / t_batch / 1e9, 3), "unit": "Gcell-steps/s",
             "ms_per_batch_gradient": round(1000 * t_batch, 2),
             "sequential_ms": round(1000 * t_seq, 2), "batch_speedup_vs_sequential": round(t_seq / t_batch, 3),
+            "sequential_fd_launch": seq_shapes,
             "models": [[p["nz"], p["nx"], p["nt"], len(p["sx"])] for p, _ in models],
             "fd_launch": {"fwd": list(tune["fwd"]), "adj": list(tune["adj"]), "candidates": tune["candidates"],
                           "tune_s": round(tune_s, 2)},

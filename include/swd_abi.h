@@ -264,9 +264,11 @@ int swd_engine_set_tile_rows(swd_engine *e, int tz);
 int swd_engine_set_time_block(swd_engine *e, int nt_block, int rows);
 int swd_engine_set_phase_time_block(swd_engine *e, int phase, int nt_block, int rows);
 /* FD autotuner (no reference counterpart): times every time block 1..6 with
- * band heights that split the tallest grid evenly, over `steps` (<= 0: 60)
- * forward and adjoint steps of the batch on its own buffers, and installs the
- * fastest forward and adjoint settings on every engine.  choice[4] =
+ * band heights that split the tallest grid evenly (for a batch of several
+ * engines: with band rows 0, chosen from occupancy for each launch table),
+ * over `steps` (<= 0: 60) forward and adjoint steps of the batch on its own
+ * buffers, and installs the fastest forward and adjoint settings on every
+ * engine.  choice[4] =
  * {forward steps/launch, forward band rows, adjoint steps/launch, adjoint
  * band rows}; table[table_rows][4] = {steps/launch, rows, forward us/step,
  * adjoint us/step} per candidate; *n_rows = candidates timed.  Overwrites FD

@@ -1,6 +1,8 @@
 // Autotuner of the fused FD march: picks the time block (steps per launch)
 // and band height of the time-blocked kernels by timing them on the batch
-// they will run, separately for the forward and the adjoint phase.
+// they will run, separately for the forward and the adjoint phase (for a
+// batch of several models, the time block only: band heights then come from
+// occupancy, launch table by launch table).
 //
 // The best setting depends on the batch: a launch is fastest when its waves
 // fill the resident-wave slots of the GPU in whole rounds while each wave
@@ -63,8 +65,16 @@ int autotune_fd_batch(const std::vector<Engine*>& es, int steps, std::vector<Tun
         r.rows = tz;
         cand.push_back(r);
     }
+    // A batch of several models changes its job set as models finish, and
+    // every launch table after the first takes its band height from
+    // occupancy (auto_band_rows); the tuner then picks only the time block,
+    // each candidate at those band heights (rows 0).  Tuning the first
+    // table's height on the first 60 steps picked 18-row forward bands for
+    // the 8-model uneven batch, where the occupancy choice (36) ran the whole
+    // batch gradient in 324 instead of 340 ms (means of two processes each).
+    const bool batch = es.size() > 1;
     for (int NT = 2; NT <= 6; ++NT)
-        for (int H : band_candidates(zmax)) {
+        for (int H : batch ? std::vector<int>{0} : band_candidates(zmax)) {
             TuneRow r;
             r.nt_block = NT;
             r.rows = H;

@@ -212,9 +212,11 @@ class Engine:
 def batch_autotune(engines, steps: int = 60) -> dict:
     """Autotune the FD march of a batch of engines: every time block 1..6 and
     band heights splitting the tallest grid evenly are timed over `steps`
-    forward and adjoint steps on the engines' own buffers; the fastest
-    forward and adjoint settings are installed on every engine.  Overwrites
-    FD state, gathers, snapshots, adjoint sources and images."""
+    forward and adjoint steps on the engines' own buffers (several engines:
+    the time blocks only, band heights from occupancy per launch table,
+    reported as rows 0); the fastest forward and adjoint settings are
+    installed on every engine.  Overwrites FD state, gathers, snapshots,
+    adjoint sources and images."""
     n = len(engines)
     arr = (ctypes.c_void_p * n)(*[e.h.value for e in engines])
     choice = np.zeros(4, dtype=np.int32)

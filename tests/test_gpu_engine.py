@@ -271,7 +271,13 @@ def test_uneven_batch_autotuned_is_the_same_alone_and_in_mix():
         e.set_model(mod["vp"], mod["vs"])
         engs.append(e)
     tune = batch_autotune(engs, steps=60)
-    assert tune["candidates"] > 10
+    # a batch tunes the time block only: one single-step candidate and time
+    # blocks 2..6 at band heights chosen from occupancy (rows 0)
+    tab = tune["table"]
+    assert tune["candidates"] == tab.shape[0] == 6
+    assert list(tab[:, 0]) == [1, 2, 3, 4, 5, 6] and np.all(tab[1:, 1] == 0)
+    for ph in ("fwd", "adj"):
+        assert tune[ph][1] == 0 or tune[ph][0] == 1
     outs, us = batch_gradient(engs)
     assert us[0] > 0 and us[1] > 0
     for (prob, mod), e, m in zip(models, engs, outs):
