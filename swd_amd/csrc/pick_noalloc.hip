@@ -82,23 +82,32 @@ static int lhdisppick_noalloc_impl(const float* ml, int np, int npair, double vm
     std::lock_guard<std::mutex> lock(g_noalloc_lock[dev]);
     hipStream_t st = hipStreamPerThread;
     const DispCol c{1, 0, 0, 1, np};
-    SWD_HIP(hipMemcpyAsync(base, &c, sizeof c, hipMemcpyHostToDevice, st));
-    SWD_HIP(hipMemcpyAsync(base + off_cr0, cr_0, (size_t)npair * sizeof(double), hipMemcpyHostToDevice, st));
-    SWD_HIP(hipMemcpyAsync(base + off_ml, ml, (size_t)np * npair * sizeof(float), hipMemcpyHostToDevice, st));
-    const float* dml = (const float*)(base + off_ml);
-    const double* dcr0 = (const double*)(base + off_cr0);
-    double* dpre = (double*)(base + off_pre);
-    if (method == 1) {
-        if ((rc = launch_pick(st, dml, nullptr, (const DispCol*)base, 1, np, npair, ini, 2.f, 4.f, vmin, 0, 0, dcr0,
-                              dpre, (double*)(base + off_res), (int*)(base + off_stat))))
-            return rc;
-    } else {
-        if ((rc = launch_argmax_pick(st, dml, np, npair, vmin, dcr0, dpre))) return rc;
-    }
     int stv = 0;
-    SWD_HIP(hipMemcpyAsync(cr_pre, dpre, (size_t)npair * sizeof(double), hipMemcpyDeviceToHost, st));
-    if (method == 1) SWD_HIP(hipMemcpyAsync(&stv, base + off_stat, sizeof(int), hipMemcpyDeviceToHost, st));
-    SWD_HIP(hipStreamSynchronize(st));
+    // everything queued below is waited for before the lock is released, on
+    // the error paths too: the next call reuses the array, and the copies read
+    // the caller's buffers
+    rc = [&]() -> int {
+        SWD_HIP(hipMemcpyAsync(base, &c, sizeof c, hipMemcpyHostToDevice, st));
+        SWD_HIP(hipMemcpyAsync(base + off_cr0, cr_0, (size_t)npair * sizeof(double), hipMemcpyHostToDevice, st));
+        SWD_HIP(hipMemcpyAsync(base + off_ml, ml, (size_t)np * npair * sizeof(float), hipMemcpyHostToDevice, st));
+        const float* dml = (const float*)(base + off_ml);
+        const double* dcr0 = (const double*)(base + off_cr0);
+        double* dpre = (double*)(base + off_pre);
+        int r;
+        if (method == 1) {
+            if ((r = launch_pick(st, dml, nullptr, (const DispCol*)base, 1, np, npair, ini, 2.f, 4.f, vmin, 0, 0,
+                                 dcr0, dpre, (double*)(base + off_res), (int*)(base + off_stat))))
+                return r;
+        } else if ((r = launch_argmax_pick(st, dml, np, npair, vmin, dcr0, dpre))) {
+            return r;
+        }
+        SWD_HIP(hipMemcpyAsync(cr_pre, dpre, (size_t)npair * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (method == 1) SWD_HIP(hipMemcpyAsync(&stv, base + off_stat, sizeof(int), hipMemcpyDeviceToHost, st));
+        return kOk;
+    }();
+    const hipError_t se = hipStreamSynchronize(st);
+    if (rc) return rc;
+    if (se != hipSuccess) { set_error("pick stream: %s", hipGetErrorString(se)); return kErrHip; }
     if (stv) { set_error("interp1 needs at least two kept pick columns"); return kErrPick; }
     double acc = 0;
     for (int i = 0; i < npair; ++i) { const double d = cr_pre[i] - cr_0[i]; acc += d * d; }
