@@ -178,9 +178,11 @@ def progress(msg):
 
 def uneven_leg(steps=2, graph=True):
     """The uneven multi-model batch: 8 faulted layered models of different grid
-    size and nt (16 shots each) in shared FD launches, launch shape chosen by
-    the autotuner on the batch.  Also times the same 8 gradients one model
-    after another and checks each model's gradient alone == in the mix."""
+    size and nt (16 shots each) in shared FD launches, steps per launch chosen
+    by the autotuner on the batch (band heights from occupancy per launch
+    table).  Also times the same 8 gradients one model after another, each at
+    the launch shape the autotuner picks for it alone, and checks each
+    model's gradient alone == in the mix."""
     from swd_amd import problems as P
     from swd_amd.engine import Engine, batch_autotune, batch_gradient
     models = P.uneven_models()
