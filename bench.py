@@ -84,9 +84,9 @@ def cpu_baseline(prob, mod, max_threads=16):
     """The FP32 statement-structure C port of the MEX time loops (oracle/):
     forward + adjoint of production-grid shots at the full nt, one shot per
     thread (a parpool worker of the reference is single-threaded), on the
-    host cores this job may use, at most `max_threads`: the GPU box gives
-    one GPU's job a 16-core share whatever nproc reports.  Also times one
-    shot alone on one core."""
+    host cores this job may use, at most `max_threads`: a one-GPU job gets a
+    16-core share of the host whatever nproc reports.  Also times one shot
+    alone on one core."""
     from oracle import swd_oracle as O
     L = O.lib()
     try:
