@@ -325,7 +325,9 @@ int swd_batch_gradient(swd_engine **engines, int n, double **dk_vs, double *kern
 /* HBM counter calibration (no reference counterpart): copies `bytes` between
  * two device buffers `reps` times with a plain vector copy kernel; *ms = mean
  * event-timed duration of one copy.  Its rocprofv3 FETCH_SIZE / WRITE_SIZE
- * readings scale the FD kernels' readings to bytes. */
+ * readings scale the FD kernels' readings to bytes.  Both copy functions
+ * check the destination against the source after the timed copies and
+ * return SWD_ERR_HIP if any 16-byte element differs. */
 int swd_calib_copy(long long bytes, int reps, double *ms);
 
 /* Practical HBM ceiling (no reference counterpart): the fastest device copy

@@ -40,6 +40,31 @@ __global__ __launch_bounds__(256) void copy_peak_kernel(const float4* __restrict
     }
 }
 
+// Source pattern of the copies: every 16-byte element distinct, so an element
+// a copy skipped or wrote to the wrong place is seen by copy_check.
+__device__ inline uint4 copy_pattern(size_t i)
+{
+    const unsigned lo = (unsigned)i, hi = (unsigned)(i >> 32);
+    return make_uint4(lo, ~lo, lo * 2654435761u + hi, hi ^ 0x5a5a5a5au);
+}
+
+__global__ __launch_bounds__(256) void copy_fill(uint4* __restrict__ a, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) a[i] = copy_pattern(i);
+}
+
+// bad is set to 1 by any lane whose destination element differs from the
+// pattern (a plain store: every writer writes the same value)
+__global__ __launch_bounds__(256) void copy_check(const uint4* __restrict__ b, size_t n, int* __restrict__ bad)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint4 v = b[i], w = copy_pattern(i);
+        if (v.x != w.x || v.y != w.y || v.z != w.z || v.w != w.w) *bad = 1;
+    }
+}
+
 }  // namespace swd
 
 using namespace swd;
@@ -49,10 +74,15 @@ static int time_copies(long long bytes, int reps, double* ms, F launch)
 {
     if (bytes < 16 || reps < 1) { set_error("copy needs bytes >= 16 and reps >= 1"); return kErrArg; }
     const size_t n = (size_t)bytes / sizeof(float4);
-    DevBuf a, b;
+    DevBuf a, b, bad;
     int rc;
-    if ((rc = a.alloc(n * sizeof(float4))) || (rc = b.alloc(n * sizeof(float4)))) return rc;
-    SWD_HIP(hipMemset(a.p, 0, a.bytes));
+    if ((rc = a.alloc(n * sizeof(float4))) || (rc = b.alloc(n * sizeof(float4))) || (rc = bad.alloc(sizeof(int))))
+        return rc;
+    const int fill_blocks = (int)std::min<size_t>((n + 255) / 256, 256 * 64);
+    copy_fill<<<fill_blocks, 256>>>(a.as<uint4>(), n);
+    SWD_HIP(hipGetLastError());
+    SWD_HIP(hipMemset(b.p, 0xff, b.bytes));
+    SWD_HIP(hipMemset(bad.p, 0, sizeof(int)));
     hipEvent_t e0, e1;
     SWD_HIP(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -70,6 +100,16 @@ static int time_copies(long long bytes, int reps, double* ms, F launch)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc != kOk) { set_error("copy kernel or its timing events failed"); return rc; }
+    // the timed copies must have moved every element: a timing of a copy
+    // that skipped part of the buffer would overstate the rate
+    copy_check<<<fill_blocks, 256>>>(b.as<uint4>(), n, bad.as<int>());
+    SWD_HIP(hipGetLastError());
+    int nbad = 1;
+    SWD_HIP(hipMemcpy(&nbad, bad.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (nbad) {
+        set_error("the copy kernel left part of its %zu-byte destination unwritten or wrong", n * sizeof(float4));
+        return kErrHip;
+    }
     if (ms) *ms = t / reps;
     return kOk;
 }
