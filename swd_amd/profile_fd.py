@@ -17,6 +17,9 @@ time-blocked kernel, and its VALU instruction count per launch when the
 benchmark; records of earlier runs at other launch shapes are kept, keyed by
 kernel and shape, e.g. ``fd_fwd_tb:6x26``) and a markdown table to stdout.
 ``--shapes`` names the shapes of the run (default: both phases at 5 x 26).
+``trace <dir>`` reads a kernel trace of a benchmark run and prints the mean,
+min and max duration of the last dispatches of each time-blocked kernel, the
+ones of the timed steps.
 """
 from __future__ import annotations
 
@@ -146,6 +149,32 @@ def summarize(args):
     print("\n".join(lines))
 
 
+def trace(args):
+    """Durations of the last `--last` dispatches of each time-blocked FD
+    kernel in a rocprofv3 kernel trace, in start order: the dispatches of the
+    timed steps of a benchmark run, whose autotuning launches come earlier."""
+    files = glob.glob(os.path.join(args.dir, "**", "*kernel_trace.csv"), recursive=True)
+    if not files:
+        raise FileNotFoundError(f"no kernel_trace.csv under {args.dir}")
+    rows = []
+    for f in files:
+        with open(f, newline="") as fh:
+            for r in csv.DictReader(fh):
+                s = _short(r["Kernel_Name"])
+                if s in ("fd_fwd_tb", "fd_adj_tb"):
+                    t0, t1 = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+                    rows.append((t0, s, (t1 - t0) / 1e3))
+    rows.sort()
+    out = {}
+    for k, n in (("fd_fwd_tb", args.last_fwd), ("fd_adj_tb", args.last_adj)):
+        us = np.array([d for _, s, d in rows if s == k][-n:])
+        out[k] = {"dispatches": int(us.size), "of": sum(1 for _, s, _ in rows if s == k),
+                  "mean_us": round(float(us.mean()), 2) if us.size else None,
+                  "min_us": round(float(us.min()), 2) if us.size else None,
+                  "max_us": round(float(us.max()), 2) if us.size else None}
+    print(json.dumps(out))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -159,8 +188,12 @@ def main(argv=None):
     s = sub.add_parser("summarize")
     s.add_argument("dir")
     s.add_argument("--shapes", default='{"fd_fwd_tb": [5, 26], "fd_adj_tb": [5, 26]}')
+    t = sub.add_parser("trace")
+    t.add_argument("dir")
+    t.add_argument("--last-fwd", type=int, default=6000, help="forward dispatches to keep (5 steps x 1200)")
+    t.add_argument("--last-adj", type=int, default=5995, help="adjoint dispatches to keep (5 steps x 1199)")
     a = ap.parse_args(argv)
-    (run if a.cmd == "run" else summarize)(a)
+    {"run": run, "summarize": summarize, "trace": trace}[a.cmd](a)
 
 
 if __name__ == "__main__":
