@@ -459,6 +459,9 @@ def main():
         tune = eng.autotune()
         tuned = {"fwd": list(tune["fwd"]), "adj": list(tune["adj"]), "candidates": tune["candidates"],
                  "seconds": round(time.perf_counter() - t0, 2)}
+        tt = eng.timings()                 # the snapshot placements the autotuner timed, and the one kept
+        tuned["snapshot_placement_ms"] = tt["snapshot_placement_ms"]
+        tuned["snapshot_placement_kept"] = tt["snapshot_placement_kept"]
     eng.set_graph(not args.no_graph)       # FD phases replay HIP graphs, captured at the first gradient
     for _ in range(args.warmup):
         eng.gradient()

@@ -281,7 +281,13 @@ int swd_engine_info(swd_engine *e, long long *vals, int n);
 /* ms[0..9] = forward, dispersion, adjoint, assembly (ms) of the last gradient,
  * mean forward / adjoint march-kernel launch duration (us, event-timed
  * sample), forward / adjoint time steps per launch, forward / adjoint band
- * rows */
+ * rows; ms[10..14] = snapshot placements swd_engine_autotune timed, the index
+ * of the one kept, and the full forward + adjoint march time (ms) of each.
+ * The adjoint's launch time depends on where the snapshot buffer lands in
+ * physical memory, so the autotuner of a single engine times one full march
+ * on its buffer and on up to two more, allocated while the earlier ones are
+ * held (each only while 8 GiB of device memory stays free beside it), and
+ * keeps the fastest. */
 int swd_engine_timings(swd_engine *e, double *ms, int n);
 /* Copy a device buffer to host: 0 gathers [shot][t][g] f32, 1 adjoint sources
  * [shot][t][g] f32, 2 {dk_vs, sum cl_img, sum cm_img, sum illum} nz x nx f64,

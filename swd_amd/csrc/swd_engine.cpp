@@ -954,9 +954,12 @@ int swd_engine_timings(swd_engine* e, double* ms, int n)
 {
     if (!e || !ms) return kErrArg;
     const StageTimes& t = e->e.times;
-    const double v[10] = {t.fwd_ms, t.disp_ms, t.adj_ms, t.asm_ms, t.fwd_kernel_us, t.adj_kernel_us,
-                          (double)t.fwd_spl, (double)t.adj_spl, (double)t.fwd_rows, (double)t.adj_rows};
-    for (int i = 0; i < n && i < 10; ++i) ms[i] = v[i];
+    const Engine& g = e->e;
+    const double v[15] = {t.fwd_ms, t.disp_ms, t.adj_ms, t.asm_ms, t.fwd_kernel_us, t.adj_kernel_us,
+                          (double)t.fwd_spl, (double)t.adj_spl, (double)t.fwd_rows, (double)t.adj_rows,
+                          (double)g.snap_placements, (double)g.snap_choice, g.snap_placement_ms[0],
+                          g.snap_placement_ms[1], g.snap_placement_ms[2]};
+    for (int i = 0; i < n && i < 15; ++i) ms[i] = v[i];
     return kOk;
 }
 

@@ -69,6 +69,10 @@ public:
     DevBuf stats;     // 4 doubles: min, max, sum of squares (model_stats)
     DispPlan disp;
     StageTimes times;
+    // snapshot placements the autotuner timed (full forward + adjoint march,
+    // ms each) and the index of the one kept (0: the first allocation)
+    float snap_placement_ms[3] = {0, 0, 0};
+    int snap_placements = 0, snap_choice = 0;
     int timing_samples = 8;
 
     ~Engine();

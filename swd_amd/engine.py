@@ -187,10 +187,15 @@ class Engine:
         return batch_autotune([self], steps)
 
     def timings(self) -> dict:
-        v = np.zeros(10)
-        check(lib().swd_engine_timings(self.h, ptr(v), 10))
-        return dict(zip(("fwd_ms", "disp_ms", "adj_ms", "asm_ms", "fwd_kernel_us", "adj_kernel_us",
-                         "fwd_steps_per_launch", "adj_steps_per_launch", "fwd_band_rows", "adj_band_rows"), v))
+        v = np.zeros(15)
+        check(lib().swd_engine_timings(self.h, ptr(v), 15))
+        out = dict(zip(("fwd_ms", "disp_ms", "adj_ms", "asm_ms", "fwd_kernel_us", "adj_kernel_us",
+                        "fwd_steps_per_launch", "adj_steps_per_launch", "fwd_band_rows", "adj_band_rows"), v))
+        # snapshot placements timed by autotune(): full forward + adjoint march ms each, and the one kept
+        n = int(v[10])
+        out["snapshot_placement_ms"] = [round(float(x), 2) for x in v[12:12 + n]]
+        out["snapshot_placement_kept"] = int(v[11])
+        return out
 
     def read(self, what: str) -> np.ndarray:
         q = self.problem

@@ -89,6 +89,10 @@ def test_engine_gradient_bitwise_across_time_blocks():
     g = e.gradient()
     t = e.timings()
     assert t["fwd_steps_per_launch"] == tune["fwd"][0] and t["adj_steps_per_launch"] == tune["adj"][0]
+    # the snapshot buffer it kept is the fastest placement it timed; the
+    # gradient on it is bitwise the same
+    pl, kept = t["snapshot_placement_ms"], t["snapshot_placement_kept"]
+    assert 1 <= len(pl) <= 3 and min(pl) > 0 and pl[kept] == min(pl)
     np.testing.assert_array_equal(e.read("gathers"), gath1)
     np.testing.assert_array_equal(g["cr_r"], g1["cr_r"])
     np.testing.assert_array_equal(g["dk_vs"], g1["dk_vs"])
